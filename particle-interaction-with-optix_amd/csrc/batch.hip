@@ -1,0 +1,281 @@
+// batch.hip -- many independent systems in one pipeline of launches.
+//
+// The reference runs one system per executeSimulation / core call (each with its own
+// allocations, launches and host synchronisations, Cuda-version/cudaLJ.cu:515-677).
+// Here the cell grids of all systems are concatenated into one key space: system s
+// owns global cells [cell_base[s], cell_base[s+1]).  One histogram, one scan, one
+// radix sort and one force launch then cover every system, whatever their sizes,
+// and one more kernel reduces each system's totals in a fixed order.  Nothing in a
+// steady-state call allocates or synchronises with the host, so the whole call can
+// be captured in a hipGraph and replayed.
+#include <cstring>
+#include <vector>
+
+#include "lj_internal.h"
+#include "lj_pair.h"
+
+namespace lj {
+
+namespace {
+
+struct SysDesc {
+    GridGeom g;
+    PairConst c;
+    uint32_t first, n, cell_base, pad;
+};
+
+// Largest s with base[s] <= v (base ascending, base[0] = 0).
+__device__ inline uint32_t find_sys(const uint32_t* __restrict__ base, uint32_t nsys, uint32_t v) {
+    uint32_t lo = 0, hi = nsys;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_batch_keys(const float* __restrict__ x, const float* __restrict__ y,
+                                                    const float* __restrict__ z, uint32_t n,
+                                                    const SysDesc* __restrict__ sys, const uint32_t* __restrict__ first,
+                                                    uint32_t nsys, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals, uint32_t* __restrict__ counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = find_sys(first, nsys, i);
+    const SysDesc& d = sys[s];
+    const uint32_t k = d.cell_base + cell_key(d.g, x[i], y[i], z[i]);
+    keys[i] = k;
+    vals[i] = i;
+    atomicAdd(&counts[k], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __restrict__ off, uint32_t ncells,
+                                                          const SysDesc* __restrict__ sys,
+                                                          const uint32_t* __restrict__ cbase, uint32_t nsys,
+                                                          lj_result* __restrict__ res) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const uint32_t nh = off[c + 1] - off[c];
+    if (!nh) return;
+    const uint32_t s = find_sys(cbase, nsys, c);
+    const GridGeom& g = sys[s].g;
+    const uint32_t base = sys[s].cell_base, local = c - base;
+    const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
+    const int32_t cx = (int32_t)(local % (uint32_t)gx);
+    const int32_t cy = (int32_t)((local / (uint32_t)gx) % (uint32_t)gy);
+    const int32_t cz = (int32_t)(local / ((uint32_t)gx * (uint32_t)gy));
+    const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
+    unsigned long long sum = 0;
+    for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
+        if (zz < 0 || zz >= gz) continue;
+        for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
+            if (yy < 0 || yy >= gy) continue;
+            const uint32_t row = base + (uint32_t)((zz * gy + yy) * gx);
+            sum += off[row + x1 + 1] - off[row + x0];
+        }
+    }
+    atomicAdd(&res[s].n_candidates, (unsigned long long)nh * (sum - 1));
+}
+
+// One thread per sorted slot, as the single-system kernel, with the grid and pair
+// constants of the slot's system.  Per-particle energy, virial and pair count go to
+// arrays in the caller's order for the per-system reduction.
+template <int PRED, int PREC>
+__global__ __launch_bounds__(256) void k_batch_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
+                                                     const uint32_t* __restrict__ off, uint32_t n,
+                                                     const SysDesc* __restrict__ sys, const uint32_t* __restrict__ cbase,
+                                                     uint32_t nsys, float* __restrict__ fx, float* __restrict__ fy,
+                                                     float* __restrict__ fz, float* __restrict__ e,
+                                                     double* __restrict__ vir, uint32_t* __restrict__ npair) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t key = cell_of[k];
+    const uint32_t s = find_sys(cbase, nsys, key);
+    const GridGeom& g = sys[s].g;
+    const PairConst c = sys[s].c;
+    const uint32_t base = sys[s].cell_base, local = key - base;
+    PairSums<PREC, PRED == LJ_PRED_CUDALJ, true> acc;
+    const float4 pi = pos[k];
+    acc.set_i(pi);
+    const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
+    const int32_t cx = (int32_t)(local % (uint32_t)gx);
+    const int32_t cy = (int32_t)((local / (uint32_t)gx) % (uint32_t)gy);
+    const int32_t cz = (int32_t)(local / ((uint32_t)gx * (uint32_t)gy));
+    const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
+    uint32_t np = 0;
+    for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
+        if (zz < 0 || zz >= gz) continue;
+        for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
+            if (yy < 0 || yy >= gy) continue;
+            const uint32_t row = base + (uint32_t)((zz * gy + yy) * gx);
+            const uint32_t b = off[row + x0], en = off[row + x1 + 1];
+            for (uint32_t t = b; t < en; ++t) {
+                const float4 pj = pos[t];
+                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const float d2 = dist2(dx, dy, dz);
+                if (t == k || !accept<PRED>(c, d2)) continue;
+                ++np;
+                acc.add(c, pj, dx, dy, dz, d2);
+            }
+        }
+    }
+    const uint32_t i = __float_as_uint(pi.w);
+    fx[i] = (float)(-c.e24 * (double)acc.ax.value());
+    fy[i] = (float)(-c.e24 * (double)acc.ay.value());
+    fz[i] = (float)(-c.e24 * (double)acc.az.value());
+    e[i] = (float)(c.e4 * (double)acc.ae.value());
+    vir[i] = c.e24 * (double)acc.av.value();
+    npair[i] = np;
+}
+
+// One block per system: FP64 totals over its particles in index order.
+__global__ __launch_bounds__(256) void k_batch_reduce(const SysDesc* __restrict__ sys, const float* __restrict__ e,
+                                                      const double* __restrict__ vir, const uint32_t* __restrict__ npair,
+                                                      lj_result* __restrict__ res) {
+    const SysDesc& d = sys[blockIdx.x];
+    double a = 0.0, b = 0.0;
+    unsigned long long np = 0;
+    for (uint32_t q = threadIdx.x; q < d.n; q += 256) {
+        a += (double)e[d.first + q];
+        b += vir[d.first + q];
+        np += npair[d.first + q];
+    }
+    block_sum2(a, b);
+    const unsigned long long t = block_sum_u64(np);
+    if (threadIdx.x == 0) {
+        res[blockIdx.x].e_total = 0.5 * a;
+        res[blockIdx.x].virial = 0.5 * b;
+        res[blockIdx.x].n_pairs = t;
+    }
+}
+
+}  // namespace
+
+}  // namespace lj
+
+using namespace lj;
+
+extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t nsys, const float* x, const float* y,
+                                  const float* z, float* fx, float* fy, float* fz, float* e, lj_result* results,
+                                  void* stream) {
+    if (!ctx) return LJ_ERR_INVALID_ARG;
+    if (nsys < 0 || (nsys > 0 && (!sys || !results))) return fail(ctx, LJ_ERR_INVALID_ARG, "bad system array");
+    if (nsys == 0) return LJ_OK;
+    // Plan: per-system geometry, contiguous particle ranges, concatenated cells.
+    std::vector<SysDesc> desc((size_t)nsys);
+    std::vector<uint32_t> first((size_t)nsys), cbase((size_t)nsys);
+    uint64_t ntot = 0, ctot = 0;
+    for (int32_t s = 0; s < nsys; ++s) {
+        GridGeom g;
+        PairConst c;
+        LJ_TRY(make_geom(ctx, &sys[s].params, sys[s].n, &g, &c));
+        if (sys[s].params.predicate != sys[0].params.predicate || sys[s].params.precision != sys[0].params.precision)
+            return fail(ctx, LJ_ERR_INVALID_ARG, "all systems of a batch must share predicate and precision");
+        if (sys[s].first != (int64_t)ntot)
+            return fail(ctx, LJ_ERR_INVALID_ARG, "system %d starts at %lld; systems must be packed back to back from 0",
+                        s, (long long)sys[s].first);
+        SysDesc& d = desc[(size_t)s];
+        std::memset(&d, 0, sizeof d);
+        d.g = g;
+        d.c = c;
+        d.first = (uint32_t)ntot;
+        d.n = (uint32_t)sys[s].n;
+        d.cell_base = (uint32_t)ctot;
+        first[(size_t)s] = d.first;
+        cbase[(size_t)s] = d.cell_base;
+        ntot += (uint64_t)sys[s].n;
+        ctot += g.ncells;
+        if (ntot > 0x7fffffffull) return fail(ctx, LJ_ERR_OVERFLOW, "batch holds more than 2^31-1 particles");
+        if (ctot > 0x7fffffffull) return fail(ctx, LJ_ERR_OVERFLOW, "batch holds more than 2^31-1 cells");
+    }
+    if (ntot > 0 && (!x || !y || !z || !fx || !fy || !fz))
+        return fail(ctx, LJ_ERR_INVALID_ARG, "positions and forces must not be NULL");
+    LJ_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    BatchWork& W = ctx->batch;
+    BinWork& B = ctx->bin;
+
+    // Upload the plan only when it changed (a cached plan keeps replays free of
+    // host-to-device copies).
+    const size_t dbytes = desc.size() * sizeof(SysDesc), ubytes = (size_t)nsys * 4;
+    std::vector<uint8_t> blob(dbytes + 2 * ubytes);
+    std::memcpy(blob.data(), desc.data(), dbytes);
+    std::memcpy(blob.data() + dbytes, first.data(), ubytes);
+    std::memcpy(blob.data() + dbytes + ubytes, cbase.data(), ubytes);
+    if (blob != W.plan) {
+        LJ_HIP(ctx, W.sys.grow(blob.size()));
+        LJ_HIP(ctx, hipMemcpyAsync(W.sys.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+        LJ_HIP(ctx, hipStreamSynchronize(s));
+        W.plan = std::move(blob);
+    }
+    const SysDesc* dsys = W.sys.as<SysDesc>();
+    const uint32_t* dfirst = reinterpret_cast<const uint32_t*>(W.sys.as<uint8_t>() + dbytes);
+    const uint32_t* dcbase = reinterpret_cast<const uint32_t*>(W.sys.as<uint8_t>() + dbytes + ubytes);
+
+    const int64_t n = (int64_t)ntot;
+    const uint32_t ncells = (uint32_t)ctot;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    LJ_HIP(ctx, B.keys[0].grow(nn * 4));
+    LJ_HIP(ctx, B.keys[1].grow(nn * 4));
+    LJ_HIP(ctx, B.vals[0].grow(nn * 4));
+    LJ_HIP(ctx, B.vals[1].grow(nn * 4));
+    LJ_HIP(ctx, B.counts.grow((size_t)ncells * 4));
+    LJ_HIP(ctx, B.offsets.grow(((size_t)ncells + 1) * 4));
+    LJ_HIP(ctx, B.pos.grow(nn * sizeof(float4)));
+    LJ_HIP(ctx, W.partials.grow(nn * sizeof(double)));
+    LJ_HIP(ctx, W.part_sys.grow(nn * sizeof(uint32_t)));
+    if (!e) {
+        LJ_HIP(ctx, W.results_tmp.grow(nn * sizeof(float)));
+        e = W.results_tmp.as<float>();
+    }
+    uint32_t* counts = B.counts.as<uint32_t>();
+    const unsigned nb = div_up(n > 0 ? n : 1, 256);
+    uint32_t* off = B.offsets.as<uint32_t>();
+    {
+    PhaseTimer tb(ctx, LJ_PHASE_BIN, s);
+    fill_u32(counts, ncells, 0u, s);
+    fill_u64(reinterpret_cast<unsigned long long*>(results), (uint32_t)(4 * nsys), 0ull, s);
+    if (n > 0)
+        hipLaunchKernelGGL(k_batch_keys, dim3(nb), dim3(256), 0, s, x, y, z, (uint32_t)n, dsys, dfirst, (uint32_t)nsys,
+                           B.keys[0].as<uint32_t>(), B.vals[0].as<uint32_t>(), counts);
+    LJ_TRY(scan_u32(ctx, counts, off, ncells, off + ncells, s));
+    int bits = 0;
+    while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)ncells) ++bits;
+    LJ_TRY(radix_sort(ctx, n, bits, s));
+    LJ_TRY(gather_sorted(ctx, x, y, z, n, s));
+    }
+    hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(ncells, 256)), dim3(256), 0, s, (const uint32_t*)off, ncells,
+                       dsys, dcbase, (uint32_t)nsys, results);
+    double* vir = W.partials.as<double>();
+    uint32_t* npair = W.part_sys.as<uint32_t>();
+    const float4* pos = B.pos.as<float4>();
+    const uint32_t* cell_of = B.sorted_keys();
+    {
+    PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
+    if (n > 0) {
+#define LJ_BF(PRED, PREC)                                                                                       \
+    hipLaunchKernelGGL((k_batch_force<PRED, PREC>), dim3(nb), dim3(256), 0, s, pos, cell_of, (const uint32_t*)off, \
+                       (uint32_t)n, dsys, dcbase, (uint32_t)nsys, fx, fy, fz, e, vir, npair)
+#define LJ_BF_PREC(PRED)                                                               \
+    switch (sys[0].params.precision) {                                                 \
+    case LJ_PREC_MIXED: LJ_BF(PRED, LJ_PREC_MIXED); break;                             \
+    case LJ_PREC_FP32: LJ_BF(PRED, LJ_PREC_FP32); break;                               \
+    default: LJ_BF(PRED, LJ_PREC_FP32_COMP); break;                                    \
+    }
+        switch (sys[0].params.predicate) {
+        case LJ_PRED_AABB: LJ_BF_PREC(LJ_PRED_AABB); break;
+        case LJ_PRED_CUSTOM: LJ_BF_PREC(LJ_PRED_CUSTOM); break;
+        case LJ_PRED_CUDALJ: LJ_BF_PREC(LJ_PRED_CUDALJ); break;
+        case LJ_PRED_SPHERE: LJ_BF_PREC(LJ_PRED_SPHERE); break;
+        case LJ_PRED_TRIANGLE: LJ_BF_PREC(LJ_PRED_TRIANGLE); break;
+        }
+#undef LJ_BF_PREC
+#undef LJ_BF
+    }
+    }
+    hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)nsys), dim3(256), 0, s, dsys, (const float*)e,
+                       (const double*)vir, (const uint32_t*)npair, results);
+    LJ_HIP(ctx, hipGetLastError());
+    return LJ_OK;
+}

@@ -1,0 +1,388 @@
+// binning.hip -- cell binning on gfx950: cell keys + histogram, device-wide scan,
+// stable LSD radix sort of (cell key, original index), gather of the sorted
+// positions, and the 27-cell candidate count.
+//
+// Reference: ComputeNbParticlePerCells (Cuda-version/cudaLJ.cu:345-383),
+// PrefixFullV2 / BuildPartialPrefix_AllSize / ApplyPartialPrefixV3
+// (Cuda-version/prefixsum.hpp:46-215), InitNewCells (cudaLJ.cu:428-439),
+// MoveParticlesToNewCells (cudaLJ.cu:441-471), ComputeNbInteractions
+// (cudaLJ.cu:255-297).
+//
+// Differences in design:
+//  - the scan is reduce-then-scan with wave64 shuffles and no host round trip or
+//    per-call allocation (the reference mallocs, recurses on the host and
+//    synchronises at every level, prefixsum.hpp:195-214);
+//  - the scatter is a stable radix sort, so the order inside a cell is the input
+//    order and every output is deterministic (the reference hands out slots with
+//    atomic tickets, cudaLJ.cu:462);
+//  - the candidate count is a closed form over cells, not a pass over particles.
+#include "lj_internal.h"
+
+namespace lj {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048 values per block
+
+__device__ inline unsigned lane_id() { return __lane_id(); }
+
+// popcount(mask & lanes below this one)
+__device__ inline unsigned rank_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// ---------------------------------------------------------------- keys
+__global__ __launch_bounds__(256) void k_keys(const float* __restrict__ x, const float* __restrict__ y,
+                                              const float* __restrict__ z, uint32_t n, GridGeom g,
+                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                              uint32_t* __restrict__ counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = cell_key(g, x[i], y[i], z[i]);
+    keys[i] = k;
+    vals[i] = i;
+    atomicAdd(&counts[k], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- scan
+template <class T>
+__device__ inline T wave_incl_scan_t(T v) {
+    const unsigned lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T o = __shfl_up(v, d, kWave);
+        if (lane >= (unsigned)d) v += o;
+    }
+    return v;
+}
+
+// Block sums of kScanTile-value tiles (TI: input type, TS: sum type).
+template <class TI, class TS>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const TI* __restrict__ in, uint32_t n,
+                                                             TS* __restrict__ partials) {
+    __shared__ TS wsum[kScanThreads / kWave];
+    const uint32_t base = blockIdx.x * kScanTile;
+    TS s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t idx = base + k * kScanThreads + threadIdx.x;
+        if (idx < n) s += (TS)in[idx];
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, kWave);
+    if (lane_id() == 0) wsum[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        TS t = 0;
+        for (int w = 0; w < kScanThreads / kWave; ++w) t += wsum[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of one tile per block, offset by the scanned block partial.
+// Thread t owns the kScanItems consecutive values [t*kScanItems, (t+1)*kScanItems).
+// in may alias out when TI == TS.
+template <class TI, class TS>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI* in, TS* out, uint32_t n,
+                                                           const TS* __restrict__ block_off, TS* __restrict__ total) {
+    __shared__ TS wsum[kScanThreads / kWave];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    TS v[kScanItems];
+    TS s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? (TS)in[base + k] : (TS)0;
+        s += v[k];
+    }
+    const TS incl = wave_incl_scan_t(s);
+    const unsigned w = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    TS woff = block_off ? block_off[blockIdx.x] : (TS)0;
+    for (unsigned q = 0; q < w; ++q) woff += wsum[q];
+    TS run = woff + incl - s;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) *total = run;
+}
+
+// ---------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                            int shift, uint32_t ntiles,
+                                                            uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t idx = base + r * kSortThreads + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one tile.  Keys are taken in rounds of 256 in input order; inside
+// a round, waves come in order and lanes in order, so (round, wave, lane) is the
+// input order.  Lanes holding the same digit are found with 8 ballots (one per
+// digit bit); a lane's rank among them is a masked popcount.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t n, int shift,
+    uint32_t ntiles, const uint32_t* __restrict__ hist, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout) {
+    constexpr int kWaves = kSortThreads / kWave;
+    __shared__ uint32_t gbase[kRadix];
+    __shared__ uint32_t wcnt[kWaves][kRadix];
+    const unsigned tid = threadIdx.x;
+    const unsigned wave = tid / kWave;
+    gbase[tid] = hist[(uint64_t)tid * ntiles + blockIdx.x];
+    const uint32_t base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        if (base + r * kSortThreads >= n) break;  // uniform across the block
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        const uint32_t idx = base + r * kSortThreads + tid;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? kin[idx] : 0u;
+        const uint32_t d = (key >> shift) & (kRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const unsigned rank = rank_below(peers);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t run = gbase[tid];
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = run;
+                run += c;
+            }
+            gbase[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = wcnt[wave][d] + rank;
+            kout[dst] = key;
+            vout[dst] = vin[idx];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- gather
+__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ x, const float* __restrict__ y,
+                                                const float* __restrict__ z, const uint32_t* __restrict__ perm,
+                                                uint32_t n, float4* __restrict__ pos) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = perm[k];
+    pos[k] = make_float4(x[i], y[i], z[i], __uint_as_float(i));
+}
+
+// ---------------------------------------------------------------- candidates
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// sum over cells of n_c * (S_c - 1), S_c = the clipped 27-cell count including c;
+// equals the per-particle count of ComputeNbInteractions (cudaLJ.cu:278-292).
+__global__ __launch_bounds__(256) void k_candidates(const uint32_t* __restrict__ off, GridGeom g,
+                                                    unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long wsum[256 / kWave];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = 0;
+    if (c < g.ncells) {
+        const uint32_t nh = off[c + 1] - off[c];
+        if (nh) {
+            const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
+            const int32_t cx = (int32_t)(c % (uint32_t)gx);
+            const int32_t cy = (int32_t)((c / (uint32_t)gx) % (uint32_t)gy);
+            const int32_t cz = (int32_t)(c / ((uint32_t)gx * (uint32_t)gy));
+            const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
+            unsigned long long s = 0;
+            for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
+                if (zz < 0 || zz >= gz) continue;
+                for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
+                    if (yy < 0 || yy >= gy) continue;
+                    const uint32_t row = (uint32_t)((zz * gy + yy) * gx);
+                    s += off[row + x1 + 1] - off[row + x0];
+                }
+            }
+            v = (unsigned long long)nh * (s - 1);
+        }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    if (lane_id() == 0) wsum[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 256 / kWave; ++w) t += wsum[w];
+        if (t) atomicAdd(out, t);
+    }
+}
+
+int key_bits(uint32_t ncells) {
+    int b = 0;
+    while (b < 32 && ((uint64_t)1 << b) < (uint64_t)ncells) ++b;
+    return b;
+}
+
+}  // namespace
+
+// Workspace layout of scan_tmp: level k partial sums back to back.
+static size_t scan_levels(int64_t n, int64_t* sizes, int* nlev) {
+    size_t total = 0;
+    int l = 0;
+    int64_t m = n;
+    while (m > kScanTile) {
+        m = (m + kScanTile - 1) / kScanTile;
+        sizes[l++] = m;
+        total += (size_t)m;
+    }
+    *nlev = l;
+    return total;
+}
+
+template <class TI, class TS>
+static void scan_rec(const TI* in, TS* out, int64_t n, TS* tmp, const int64_t* sizes, int lev, int nlev,
+                     TS* total_dev, hipStream_t s) {
+    if (lev == nlev) {  // fits in one block
+        hipLaunchKernelGGL((k_scan_down<TI, TS>), dim3(1), dim3(kScanThreads), 0, s, in, out, (uint32_t)n,
+                           (const TS*)nullptr, total_dev);
+        return;
+    }
+    const unsigned nb = div_up(n, kScanTile);
+    TS* part = tmp;
+    hipLaunchKernelGGL((k_scan_reduce<TI, TS>), dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, part);
+    scan_rec<TS, TS>(part, part, sizes[lev], tmp + sizes[lev], sizes, lev + 1, nlev, (TS*)nullptr, s);
+    hipLaunchKernelGGL((k_scan_down<TI, TS>), dim3(nb), dim3(kScanThreads), 0, s, in, out, (uint32_t)n,
+                       (const TS*)part, total_dev);
+}
+
+template <class TI, class TS>
+static int scan_any(lj_context* ctx, const TI* in, TS* out, int64_t n, TS* total_dev, hipStream_t s) {
+    if (n <= 0) {
+        if (total_dev) {
+            if constexpr (sizeof(TS) == 8) fill_u64((unsigned long long*)total_dev, 1, 0ull, s);
+            else fill_u32((uint32_t*)total_dev, 1, 0u, s);
+        }
+        LJ_HIP(ctx, hipGetLastError());
+        return LJ_OK;
+    }
+    int64_t sizes[8];
+    int nlev = 0;
+    const size_t need = scan_levels(n, sizes, &nlev);
+    LJ_HIP(ctx, ctx->bin.scan_tmp.grow((need + 1) * sizeof(TS)));
+    scan_rec<TI, TS>(in, out, n, ctx->bin.scan_tmp.as<TS>(), sizes, 0, nlev, total_dev, s);
+    LJ_HIP(ctx, hipGetLastError());
+    return LJ_OK;
+}
+
+int scan_u32(lj_context* ctx, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, hipStream_t s) {
+    return scan_any<uint32_t, uint32_t>(ctx, in, out, n, total_dev, s);
+}
+
+int scan_u32_to_u64(lj_context* ctx, const uint32_t* in, unsigned long long* out, int64_t n, hipStream_t s) {
+    return scan_any<uint32_t, unsigned long long>(ctx, in, out, n, out + (n > 0 ? n : 0), s);
+}
+
+int radix_sort(lj_context* ctx, int64_t n, int bits, hipStream_t s) {
+    BinWork& w = ctx->bin;
+    const unsigned ntiles = div_up(n, kSortTile);
+    LJ_HIP(ctx, w.hist.grow((size_t)kRadix * (ntiles ? ntiles : 1) * 4));
+    int cur = 0;
+    if (n > 1) {
+        for (int shift = 0; shift < bits; shift += kRadixBits) {
+            uint32_t* hist = w.hist.as<uint32_t>();
+            hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kSortThreads), 0, s, w.keys[cur].as<uint32_t>(),
+                               (uint32_t)n, shift, ntiles, hist);
+            LJ_TRY(scan_u32(ctx, hist, hist, (int64_t)kRadix * ntiles, nullptr, s));
+            hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, s, w.keys[cur].as<uint32_t>(),
+                               w.vals[cur].as<uint32_t>(), (uint32_t)n, shift, ntiles, hist,
+                               w.keys[cur ^ 1].as<uint32_t>(), w.vals[cur ^ 1].as<uint32_t>());
+            cur ^= 1;
+        }
+    }
+    w.cur = cur;
+    LJ_HIP(ctx, hipGetLastError());
+    return LJ_OK;
+}
+
+int bin_particles(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
+                  const GridGeom& g, hipStream_t s) {
+    BinWork& w = ctx->bin;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    LJ_HIP(ctx, w.keys[0].grow(nn * 4));
+    LJ_HIP(ctx, w.keys[1].grow(nn * 4));
+    LJ_HIP(ctx, w.vals[0].grow(nn * 4));
+    LJ_HIP(ctx, w.vals[1].grow(nn * 4));
+    LJ_HIP(ctx, w.counts.grow((size_t)g.ncells * 4));
+    LJ_HIP(ctx, w.offsets.grow(((size_t)g.ncells + 1) * 4));
+    LJ_HIP(ctx, w.pos.grow(nn * sizeof(float4)));
+    uint32_t* counts = w.counts.as<uint32_t>();
+    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(g.ncells, 256)), dim3(256), 0, s, counts,
+                       (uint64_t)g.ncells, 0u);
+    if (n > 0)
+        hipLaunchKernelGGL(k_keys, dim3(div_up(n, 256)), dim3(256), 0, s, x, y, z, (uint32_t)n, g,
+                           w.keys[0].as<uint32_t>(), w.vals[0].as<uint32_t>(), counts);
+    // offsets[0..nc) = exclusive scan of counts, offsets[nc] = n
+    uint32_t* off = w.offsets.as<uint32_t>();
+    LJ_TRY(scan_u32(ctx, counts, off, g.ncells, off + g.ncells, s));
+
+    LJ_TRY(radix_sort(ctx, n, key_bits(g.ncells), s));
+    return gather_sorted(ctx, x, y, z, n, s);
+}
+
+int gather_sorted(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, hipStream_t s) {
+    BinWork& w = ctx->bin;
+    if (n > 0)
+        hipLaunchKernelGGL(k_gather, dim3(div_up(n, 256)), dim3(256), 0, s, x, y, z, w.perm(), (uint32_t)n,
+                           w.pos.as<float4>());
+    LJ_HIP(ctx, hipGetLastError());
+    return LJ_OK;
+}
+
+int count_candidates(lj_context* ctx, const GridGeom& g, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, out, 1u, 0ull);
+    hipLaunchKernelGGL(k_candidates, dim3(div_up(g.ncells, 256)), dim3(256), 0, s,
+                       ctx->bin.offsets.as<uint32_t>(), g, out);
+    LJ_HIP(ctx, hipGetLastError());
+    return LJ_OK;
+}
+
+void fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_u64, dim3(div_up(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(div_up((int64_t)n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+}  // namespace lj

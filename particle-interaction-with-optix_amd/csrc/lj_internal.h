@@ -1,0 +1,186 @@
+// lj_internal.h -- context, workspace and launch helpers shared by the HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "lj_amd.h"
+#include "lj_device.h"
+
+struct lj_context;
+
+namespace lj {
+
+// A grow-only device buffer.  grow() reallocates only when the request exceeds the
+// current capacity, so steady-state calls of one size allocate nothing.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t grow(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            if (e != hipSuccess) return e;
+            ptr = nullptr;
+            bytes = 0;
+        }
+        size_t cap = need < 256 ? 256 : need;
+        hipError_t e = hipMalloc(&ptr, cap);
+        if (e != hipSuccess) { ptr = nullptr; return e; }
+        bytes = cap;
+        return hipSuccess;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Status carrier: the first failing check records its message on the context.
+struct Status {
+    int code = LJ_OK;
+    explicit operator bool() const { return code != LJ_OK; }
+};
+
+// Workspace of one binned system (or of a concatenated batch).
+struct BinWork {
+    DevBuf keys[2];      // u32 radix keys (ping-pong)
+    DevBuf vals[2];      // u32 original index (ping-pong)
+    DevBuf hist;         // radix digit histograms
+    DevBuf scan_tmp;     // partial sums of the device-wide scans
+    DevBuf counts;       // u32 per-cell counts
+    DevBuf offsets;      // u32 per-cell offsets (ncells+1)
+    DevBuf pos;          // float4 sorted positions, w = original index bits
+    DevBuf cell_of;      // u32 cell key per sorted slot
+    DevBuf partials;     // per-block partial sums (double) for deterministic totals
+    DevBuf counters;     // u64 counters
+    DevBuf rows;         // u64 pair-list row offsets
+    int cur = 0;         // which keys/vals buffer holds the sorted result
+    const uint32_t* sorted_keys() const { return keys[cur].as<uint32_t>(); }
+    const uint32_t* perm() const { return vals[cur].as<uint32_t>(); }
+};
+
+struct LbvhWork {
+    DevBuf morton;        // u32/u64 keys
+    DevBuf order[2];
+    DevBuf node_box;      // float4 x2 per node (lo, hi)
+    DevBuf node_child;    // int2 per internal node
+    DevBuf parent;        // int per node
+    DevBuf flags;         // refit arrival counters
+    DevBuf leaf_pos;      // float4 leaf positions in Morton order
+};
+
+struct BatchWork {
+    DevBuf sys;           // device copy of per-system descriptors
+    DevBuf cell_base;     // u32 first global cell of each system
+    DevBuf part_sys;      // u32 system of each particle
+    DevBuf tiles;         // work items
+    DevBuf partials;
+    DevBuf results_tmp;
+    std::vector<uint8_t> plan;  // host copy of the plan last uploaded to sys
+};
+
+}  // namespace lj
+
+namespace lj {
+
+// Event pairs per phase; reused across enable/read cycles.
+struct Profiler {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[LJ_NUM_PHASES];
+    size_t used[LJ_NUM_PHASES] = {0, 0, 0};
+};
+
+}  // namespace lj
+
+struct lj_context {
+    int device = 0;
+    lj::Profiler prof;
+    std::string err;
+    int num_cus = 0;
+    lj::BinWork bin;
+    lj::LbvhWork lbvh;
+    lj::BatchWork batch;
+};
+
+namespace lj {
+
+int fail(lj_context* ctx, int code, const char* fmt, ...);
+
+#define LJ_HIP(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e__ = (expr);                                                            \
+        if (e__ != hipSuccess)                                                              \
+            return ::lj::fail((ctx), e__ == hipErrorOutOfMemory ? LJ_ERR_OOM : LJ_ERR_HIP,  \
+                              "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e__),       \
+                              __FILE__, __LINE__);                                          \
+    } while (0)
+
+#define LJ_TRY(expr)              \
+    do {                          \
+        int r__ = (expr);         \
+        if (r__ != LJ_OK) return r__; \
+    } while (0)
+
+// Validates params and derives the grid and pair constants.  n is the particle
+// count of the system (for the index-width checks).
+int make_geom(lj_context* ctx, const lj_params* p, int64_t n, GridGeom* g, PairConst* c);
+
+// Device-wide exclusive scan of n u32 values (in may alias out).  Writes the total
+// to *total_dev (nullable).  Uses ctx->bin.scan_tmp.
+int scan_u32(lj_context* ctx, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev,
+             hipStream_t s);
+
+// Exclusive scan of n u32 values into u64 out[0..n), with out[n] = total.
+int scan_u32_to_u64(lj_context* ctx, const uint32_t* in, unsigned long long* out, int64_t n, hipStream_t s);
+
+// Cell-grid pipelines (force.hip).
+int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const lj_params* p,
+                     const GridGeom& g, const PairConst& c, float* fx, float* fy, float* fz, float* e,
+                     lj_result* result, hipStream_t s);
+
+// Stable LSD radix sort (8-bit digits) of ctx->bin.keys[0] with payload ctx->bin.vals[0],
+// both of n entries; sorts the low `bits` bits.  The result is in keys[cur]/vals[cur].
+int radix_sort(lj_context* ctx, int64_t n, int bits, hipStream_t s);
+
+// res->e_total / res->virial from per-block FP64 partials (pairs of doubles), fixed order.
+void finalize_energy(const double* partials, unsigned nblocks, lj_result* res, hipStream_t s);
+
+// ctx->bin.pos[k] = (x, y, z, bits of i) for i = perm[k] after radix_sort.
+int gather_sorted(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, hipStream_t s);
+
+// Bins n particles: fills ctx->bin.offsets (ncells+1), ctx->bin.pos (sorted float4 with
+// the original index in w) and ctx->bin.cell_of.
+int bin_particles(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
+                  const GridGeom& g, hipStream_t s);
+
+// Candidate count of the 27-cell stencil from ctx->bin.offsets, into *out (device u64),
+// which is overwritten (not accumulated).
+int count_candidates(lj_context* ctx, const GridGeom& g, unsigned long long* out, hipStream_t s);
+
+void fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+// Records a phase's start event on construction and its stop event on destruction
+// when profiling is on.
+class PhaseTimer {
+  public:
+    PhaseTimer(lj_context* ctx, int phase, hipStream_t s);
+    ~PhaseTimer();
+
+  private:
+    lj_context* ctx_;
+    int phase_;
+    hipStream_t s_;
+    long idx_ = -1;
+};
+
+inline unsigned div_up(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace lj

@@ -1,0 +1,145 @@
+// lj_pair.h -- per-pair Lennard-Jones arithmetic and block reductions shared by the
+// cell-grid, LBVH and batched kernels (one definition, so every path computes the
+// same terms for the same pair).
+#pragma once
+
+#include "lj_device.h"
+
+namespace lj {
+
+constexpr int kWaveSize = 64;
+
+// Per-particle accumulators of one precision mode (LJ_PREC_*).
+//   MIXED      FP64 sums of FP64 pair terms
+//   FP32       plain FP32 sums
+//   FP32_COMP  FP32 sums with the TwoSum error term kept apart (Knuth, TAOCP 2,
+//              4.2.2; Ogita, Rump, Oishi, "Accurate sum and dot product", 2005)
+template <int PREC>
+struct Acc;
+
+template <>
+struct Acc<LJ_PREC_MIXED> {
+    double s = 0.0;
+    __device__ inline void add(double v) { s += v; }
+    __device__ inline void fma(double a, double b) { s = __builtin_fma(a, b, s); }
+    __device__ inline double value() const { return s; }
+};
+
+template <>
+struct Acc<LJ_PREC_FP32> {
+    float s = 0.f;
+    __device__ inline void add(float v) { s += v; }
+    __device__ inline void fma(float a, float b) { s = __builtin_fmaf(a, b, s); }
+    __device__ inline float value() const { return s; }
+};
+
+template <>
+struct Acc<LJ_PREC_FP32_COMP> {
+    float s = 0.f, c = 0.f;
+    __device__ inline void add(float v) {
+        const float t = s + v;
+        const float bp = t - s;
+        c += (s - (t - bp)) + (v - bp);
+        s = t;
+    }
+    __device__ inline void fma(float a, float b) { add(a * b); }
+    __device__ inline float value() const { return s + c; }
+};
+
+// FP64 reciprocal: hardware estimate plus two Newton-Raphson steps.
+__device__ inline double rcp_f64(double a) {
+    double y = __builtin_amdgcn_rcp(a);
+    double e = __builtin_fma(-a, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-a, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+// Accumulates one accepted pair d = p_j - p_i into the per-particle sums:
+//   t = s6 (2 s6 - 1),  w = t / r2   (F_i = -24 eps sum w d),
+//   energy term s6 (s6 - 1)          (e_i = 4 eps sum),
+//   virial term t                    (sum_j w r2 = sum_j t).
+// s6 = (sigma^2 / r2)^3 with r2 = |d|^2 (+ softening in CUDALJ mode).  In MIXED
+// precision d is taken in FP64 from the FP32 positions, where the difference of two
+// floats is exact; in FP32 modes d is the FP32 difference used by the predicate.
+template <int PREC, bool SOFT, bool VIR>
+struct PairSums {
+    Acc<PREC> ax, ay, az, ae, av;
+    double px = 0.0, py = 0.0, pz = 0.0;  // p_i in FP64 (MIXED)
+    __device__ inline void set_i(float4 pi) {
+        px = pi.x;
+        py = pi.y;
+        pz = pi.z;
+    }
+    __device__ inline void add(const PairConst& c, float4 pj, float dx, float dy, float dz, float d2) {
+        if constexpr (PREC == LJ_PREC_MIXED) {
+            const double ddx = (double)pj.x - px, ddy = (double)pj.y - py, ddz = (double)pj.z - pz;
+            double r2 = __builtin_fma(ddz, ddz, __builtin_fma(ddy, ddy, ddx * ddx));
+            if constexpr (SOFT) r2 += c.softd;
+            const double ir2 = rcp_f64(r2);
+            const double s2 = c.sig2d * ir2;
+            const double s6 = s2 * s2 * s2;
+            const double t = __builtin_fma(s6 + s6, s6, -s6);
+            const double w = t * ir2;
+            ax.fma(w, ddx);
+            ay.fma(w, ddy);
+            az.fma(w, ddz);
+            ae.add(__builtin_fma(s6, s6, -s6));
+            if constexpr (VIR) av.add(t);
+        } else {
+            float r2 = d2;
+            if constexpr (SOFT) r2 += c.soft;
+            const float ir2 = __builtin_amdgcn_rcpf(r2);
+            const float s2 = c.sig2 * ir2;
+            const float s6 = s2 * s2 * s2;
+            const float t = __builtin_fmaf(s6 + s6, s6, -s6);
+            const float w = t * ir2;
+            ax.fma(w, dx);
+            ay.fma(w, dy);
+            az.fma(w, dz);
+            ae.add(__builtin_fmaf(s6, s6, -s6));
+            if constexpr (VIR) av.add(t);
+        }
+    }
+};
+
+template <class T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int d = kWaveSize / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWaveSize);
+    return v;
+}
+
+// Block of 256: deterministic tree sum of two doubles; thread 0 holds the result.
+__device__ inline void block_sum2(double& a, double& b) {
+    __shared__ double sh[2][256 / kWaveSize];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const unsigned w = threadIdx.x / kWaveSize;
+    if (__lane_id() == 0) {
+        sh[0][w] = a;
+        sh[1][w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = 0.0;
+        b = 0.0;
+        for (int q = 0; q < 256 / kWaveSize; ++q) {
+            a += sh[0][q];
+            b += sh[1][q];
+        }
+    }
+}
+
+__device__ inline unsigned long long block_sum_u64(unsigned long long v) {
+    __shared__ unsigned long long sh[256 / kWaveSize];
+    v = wave_sum(v);
+    if (__lane_id() == 0) sh[threadIdx.x / kWaveSize] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 256 / kWaveSize; ++q) t += sh[q];
+    return t;
+}
+
+}  // namespace lj

@@ -1,0 +1,205 @@
+"""GPU parity of the cell-grid path (binning, forces/energies, neighbour lists)
+against the CPU oracle, through the C ABI.
+
+Bars: bit-exact for cell offsets, permutation, candidate and pair counts and
+neighbour lists; relative force error <= 1e-5 against the FP64 oracle for the
+default (mixed) precision, the reference's own tolerance (Cuda-version/cudaLJ.cu:246-250).
+"""
+import numpy as np
+import pytest
+
+import ljamd
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5  # relative, per particle, vs FP64
+
+
+def _case(name):
+    if name == "c1_uniform_100k":
+        x, y, z = ljamd.generate("uniform", 1234, 100_000)
+        return (x, y, z), ljamd.make_params(0.04)
+    if name == "c2_surface_200k":
+        x, y, z = ljamd.generate("surface", 1235, 200_000)
+        return (x, y, z), ljamd.make_params(0.01, box_lo=(-0.5,) * 3, box_hi=(1.5,) * 3)
+    if name == "c5_clusters_64k":
+        x, y, z = ljamd.generate("clusters", 1238, 65_536, [16, 0.02, 0.1, 0.9])
+        return (x, y, z), ljamd.make_params(0.005)
+    if name == "tiny_7":
+        x, y, z = ljamd.generate("uniform", 3, 7)
+        return (x, y, z), ljamd.make_params(0.5)
+    if name == "single":
+        return (np.array([0.5], np.float32),) * 3, ljamd.make_params(0.1)
+    if name == "empty":
+        e = np.zeros(0, np.float32)
+        return (e, e, e), ljamd.make_params(0.1)
+    if name == "edges":
+        # exactly on cell faces, the largest float below the box edge (rounds to G),
+        # outside the box on both sides, duplicates (d2 = 0), NaN-free
+        rc = 0.04
+        vals = [0.0, 0.04, 0.08, 0.12, np.nextafter(np.float32(1.0), np.float32(0)), 1.0, 1.3, -0.2, 0.5, 0.5]
+        g = np.array(vals, np.float32)
+        xs, ys, zs = np.meshgrid(g, g, g[:4], indexing="ij")
+        return (xs.ravel(), ys.ravel(), zs.ravel()), ljamd.make_params(rc)
+    if name == "one_cell_crowd":
+        x, y, z = ljamd.generate("uniform", 11, 3000, [0.30, 0.31])
+        return (x, y, z), ljamd.make_params(0.05)
+    raise KeyError(name)
+
+
+BIN_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7", "single", "empty", "edges", "one_cell_crowd"]
+
+
+@pytest.mark.parametrize("name", BIN_CASES)
+def test_bin_bit_exact(engine, name):
+    (x, y, z), p = _case(name)
+    print(f"[bin] {name} n={x.size}", flush=True)
+    q = po.from_lj(p)
+    off_ref, perm_ref, _ = po.bin(q, x, y, z)
+    dx, dy, dz = engine.to_device(x, y, z)
+    off, perm, cand = engine.bin(dx, dy, dz, p)
+    engine.torch.cuda.synchronize()
+    off = off.cpu().numpy().view(np.uint32)
+    perm = perm.cpu().numpy().view(np.uint32)
+    assert np.array_equal(off, off_ref)
+    assert np.array_equal(perm, perm_ref)
+    assert int(cand.cpu()[0]) == po.candidates(q, off_ref) == po.candidates_direct(q, x, y, z, off_ref)
+
+
+def _forces(engine, x, y, z, p, energies=True):
+    dx, dy, dz = engine.to_device(x, y, z)
+    out = engine.compute(dx, dy, dz, p, energies=energies)
+    engine.torch.cuda.synchronize()
+    n = x.size
+    res = ljamd.read_results(out["result"])[0]
+    f = [out[k][:n].cpu().numpy() for k in ("fx", "fy", "fz")]
+    e = out["e"][:n].cpu().numpy() if energies else None
+    return f, e, res
+
+
+FORCE_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7", "single", "empty", "edges", "one_cell_crowd"]
+
+
+@pytest.mark.parametrize("name", FORCE_CASES)
+def test_forces_mixed_vs_fp64(engine, name):
+    (x, y, z), p = _case(name)
+    print(f"[force] {name} n={x.size}", flush=True)
+    q = po.from_lj(p)
+    ref = po.forces_f64(q, x, y, z)
+    f, e, res = _forces(engine, x, y, z, p)
+    assert int(res["n_pairs"]) == ref["n_pairs"]
+    assert int(res["n_candidates"]) == ref["n_candidates"]
+    if x.size == 0:
+        return
+    fin = np.isfinite(ref["fx"]) & (np.abs(ref["fx"]) < 3e38) & (np.abs(ref["fy"]) < 3e38) & (np.abs(ref["fz"]) < 3e38)
+    err = po.rel_errors([a[fin] for a in f], (ref["fx"][fin], ref["fy"][fin], ref["fz"][fin]))["rel"]
+    assert err.max() <= FORCE_TOL, (err.max(), int(np.argmax(err)))
+    efin = fin & (np.abs(ref["e"]) < 3e38)
+    eerr = np.abs(e[efin] - ref["e"][efin]) / np.maximum(np.abs(ref["e"][efin]), 1e-30)
+    assert eerr.max() <= FORCE_TOL
+    if np.all(fin):
+        assert res["e_total"] == pytest.approx(ref["e_total"], rel=1e-9, abs=1e-12)
+        assert res["virial"] == pytest.approx(ref["virial"], rel=1e-9, abs=1e-12)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-4), ("fp32_comp", 1e-4)])
+def test_forces_fp32_modes(engine, prec, tol):
+    (x, y, z), p = _case("c1_uniform_100k")
+    p.precision = ljamd.PREC[prec]
+    ref = po.forces_f64(po.from_lj(p), x, y, z)
+    f, e, res = _forces(engine, x, y, z, p)
+    assert int(res["n_pairs"]) == ref["n_pairs"]
+    err = po.rel_errors(f, (ref["fx"], ref["fy"], ref["fz"]), ref["cond"])
+    # conditioned error |F - F64| / sum_j |f_ij|: the FP32 pair chain rounds sigma^2,
+    # r2 and 1/r2 (v_rcp_f32, 1 ulp), and f ~ r^-14 amplifies those ~7x, so about
+    # 35 ulp = 2e-6 per pair; 4e-6 leaves room for the summation
+    assert err["cond"].max() <= 4e-6
+    assert np.quantile(err["rel"], 0.999) <= tol
+
+
+@pytest.mark.parametrize("pred", ["custom", "sphere", "triangle"])
+def test_compat_predicates(engine, pred):
+    x, y, z = ljamd.generate("uniform", 21, 20_000)
+    p = ljamd.make_params(0.06, predicate=pred)
+    ref = po.forces_f64(po.from_lj(p), x, y, z)
+    f, e, res = _forces(engine, x, y, z, p)
+    assert int(res["n_pairs"]) == ref["n_pairs"]
+    err = po.rel_errors(f, (ref["fx"], ref["fy"], ref["fz"]))["rel"]
+    assert err.max() <= FORCE_TOL
+
+
+def test_cudalj_energy_semantics(engine):
+    """Reference kernel semantics: every 27-cell candidate, softened r, sigma=eps=1,
+    energies compared with CheckEqual's 1e-5 relative bar (cudaLJ.cu:246-250)."""
+    x, y, z = ljamd.generate("uniform", 1234, 4096)
+    p = ljamd.make_params(0.125, sigma=1.0, epsilon=1.0, softening2=1e-5, predicate="cudalj")
+    e_ref = po.cudalj_energy(po.from_lj(p), x, y, z)
+    f, e, res = _forces(engine, x, y, z, p)
+    assert int(res["n_pairs"]) == int(res["n_candidates"])
+    rel = np.abs(e - e_ref) / np.abs(e_ref)
+    assert rel.max() <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["c1_uniform_100k", "c5_clusters_64k", "tiny_7", "edges", "one_cell_crowd"])
+def test_pairs_bit_exact(engine, name):
+    (x, y, z), p = _case(name)
+    q = po.from_lj(p)
+    rows_ref, cols_ref = po.pairs_cellgrid(q, x, y, z)
+    dx, dy, dz = engine.to_device(x, y, z)
+    rows, cols = engine.pairs(dx, dy, dz, p)
+    engine.torch.cuda.synchronize()
+    assert np.array_equal(rows.cpu().numpy().astype(np.uint64), rows_ref)
+    assert np.array_equal(cols.cpu().numpy().view(np.uint32), cols_ref)
+
+
+def test_pairs_capacity_error(engine):
+    (x, y, z), p = _case("tiny_7")
+    dx, dy, dz = engine.to_device(x, y, z)
+    rows_ref, _ = po.pairs_brute(po.from_lj(p), x, y, z)
+    need = int(rows_ref[-1])
+    assert need > 1
+    with pytest.raises(ljamd.LJError) as ei:
+        engine.pairs(dx, dy, dz, p, capacity=need - 1)
+    assert ei.value.code == -3
+
+
+def test_deterministic_bits(engine):
+    (x, y, z), p = _case("c1_uniform_100k")
+    a, ea, ra = _forces(engine, x, y, z, p)
+    b, eb, rb = _forces(engine, x, y, z, p)
+    for u, v in zip(a, b):
+        assert np.array_equal(u.view(np.uint32), v.view(np.uint32))
+    assert ra.tobytes() == rb.tobytes()
+
+
+def test_invalid_params_rejected(engine):
+    x, y, z = ljamd.generate("uniform", 1, 100)
+    dx, dy, dz = engine.to_device(x, y, z)
+    for bad in (
+        ljamd.make_params(-1.0),
+        ljamd.make_params(0.1, cell_edge=0.05),
+        ljamd.make_params(0.1, box_lo=(1, 0, 0), box_hi=(0, 1, 1)),
+        ljamd.make_params(0.1, predicate=9),
+        ljamd.make_params(1e-9),
+    ):
+        with pytest.raises(ljamd.LJError):
+            engine.compute(dx, dy, dz, bad)
+
+
+def test_c3_precision_subsample(engine):
+    """16.7M uniform particles, rc = 0.01: max/RMS force error vs FP64 on the
+    65,536-particle subsample (every 256th index); full-size invariants."""
+    n = 16_777_216
+    x, y, z = ljamd.generate("uniform", 1236, n)
+    p = ljamd.make_params(0.01)
+    f, e, res = _forces(engine, x, y, z, p, energies=True)
+    sub = np.arange(0, n, 256)
+    ref = po.forces_f64(po.from_lj(p), x, y, z, subset=sub, nthreads=16)
+    fin = (np.abs(ref["fx"]) < 3e38) & (np.abs(ref["fy"]) < 3e38) & (np.abs(ref["fz"]) < 3e38)
+    err = po.rel_errors([a[sub][fin] for a in f], (ref["fx"][fin], ref["fy"][fin], ref["fz"][fin]))["rel"]
+    print(f"[c3] max rel {err.max():.3e} rms {np.sqrt((err**2).mean()):.3e} nonfinite64 {(~fin).sum()}")
+    assert err.max() <= FORCE_TOL
+    # size-independent invariants: ordered pairs come in (i,j),(j,i) couples
+    assert int(res["n_pairs"]) % 2 == 0
+    assert int(res["n_pairs"]) == pytest.approx(70.3 * n, rel=0.02)
