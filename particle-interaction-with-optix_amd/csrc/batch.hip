@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_batch_force(const float4* __restrict__ 
                                                      const SysDesc* __restrict__ sys, const uint32_t* __restrict__ cbase,
                                                      uint32_t nsys, float* __restrict__ fx, float* __restrict__ fy,
                                                      float* __restrict__ fz, float* __restrict__ e,
-                                                     double* __restrict__ vir, uint32_t* __restrict__ npair) {
+                                                     double2* __restrict__ ev, uint32_t* __restrict__ npair) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t key = cell_of[k];
@@ -124,21 +124,22 @@ __global__ __launch_bounds__(256) void k_batch_force(const float4* __restrict__ 
     fx[i] = (float)(-c.e24 * (double)acc.ax.value());
     fy[i] = (float)(-c.e24 * (double)acc.ay.value());
     fz[i] = (float)(-c.e24 * (double)acc.az.value());
-    e[i] = (float)(c.e4 * (double)acc.ae.value());
-    vir[i] = c.e24 * (double)acc.av.value();
+    const double ei = c.e4 * (double)acc.ae.value();
+    if (e) e[i] = (float)ei;
+    ev[i] = make_double2(ei, c.e24 * (double)acc.av.value());
     npair[i] = np;
 }
 
 // One block per system: FP64 totals over its particles in index order.
-__global__ __launch_bounds__(256) void k_batch_reduce(const SysDesc* __restrict__ sys, const float* __restrict__ e,
-                                                      const double* __restrict__ vir, const uint32_t* __restrict__ npair,
-                                                      lj_result* __restrict__ res) {
+__global__ __launch_bounds__(256) void k_batch_reduce(const SysDesc* __restrict__ sys, const double2* __restrict__ ev,
+                                                      const uint32_t* __restrict__ npair, lj_result* __restrict__ res) {
     const SysDesc& d = sys[blockIdx.x];
     double a = 0.0, b = 0.0;
     unsigned long long np = 0;
     for (uint32_t q = threadIdx.x; q < d.n; q += 256) {
-        a += (double)e[d.first + q];
-        b += vir[d.first + q];
+        const double2 v = ev[d.first + q];
+        a += v.x;
+        b += v.y;
         np += npair[d.first + q];
     }
     block_sum2(a, b);
@@ -223,12 +224,8 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     LJ_HIP(ctx, B.counts.grow((size_t)ncells * 4));
     LJ_HIP(ctx, B.offsets.grow(((size_t)ncells + 1) * 4));
     LJ_HIP(ctx, B.pos.grow(nn * sizeof(float4)));
-    LJ_HIP(ctx, W.partials.grow(nn * sizeof(double)));
+    LJ_HIP(ctx, W.partials.grow(nn * sizeof(double2)));
     LJ_HIP(ctx, W.part_sys.grow(nn * sizeof(uint32_t)));
-    if (!e) {
-        LJ_HIP(ctx, W.results_tmp.grow(nn * sizeof(float)));
-        e = W.results_tmp.as<float>();
-    }
     uint32_t* counts = B.counts.as<uint32_t>();
     const unsigned nb = div_up(n > 0 ? n : 1, 256);
     uint32_t* off = B.offsets.as<uint32_t>();
@@ -247,7 +244,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     }
     hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(ncells, 256)), dim3(256), 0, s, (const uint32_t*)off, ncells,
                        dsys, dcbase, (uint32_t)nsys, results);
-    double* vir = W.partials.as<double>();
+    double2* ev = W.partials.as<double2>();
     uint32_t* npair = W.part_sys.as<uint32_t>();
     const float4* pos = B.pos.as<float4>();
     const uint32_t* cell_of = B.sorted_keys();
@@ -256,7 +253,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     if (n > 0) {
 #define LJ_BF(PRED, PREC)                                                                                       \
     hipLaunchKernelGGL((k_batch_force<PRED, PREC>), dim3(nb), dim3(256), 0, s, pos, cell_of, (const uint32_t*)off, \
-                       (uint32_t)n, dsys, dcbase, (uint32_t)nsys, fx, fy, fz, e, vir, npair)
+                       (uint32_t)n, dsys, dcbase, (uint32_t)nsys, fx, fy, fz, e, ev, npair)
 #define LJ_BF_PREC(PRED)                                                               \
     switch (sys[0].params.precision) {                                                 \
     case LJ_PREC_MIXED: LJ_BF(PRED, LJ_PREC_MIXED); break;                             \
@@ -274,8 +271,11 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
 #undef LJ_BF
     }
     }
-    hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)nsys), dim3(256), 0, s, dsys, (const float*)e,
-                       (const double*)vir, (const uint32_t*)npair, results);
+    {
+        PhaseTimer to(ctx, LJ_PHASE_OTHER, s);
+        hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)nsys), dim3(256), 0, s, dsys, (const double2*)ev,
+                           (const uint32_t*)npair, results);
+    }
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, uint32_t 
 // Bottom-up bounds.  Node boxes: box[node] = (lo.xyz, hi.xyz) as two float4.
 __global__ __launch_bounds__(256) void k_refit(const float4* __restrict__ leaf, int n, const int2* __restrict__ child,
                                                const int* __restrict__ parent, float4* __restrict__ box,
-                                               int* __restrict__ arrived) {
+                                               int* __restrict__ arrived, int* __restrict__ overflow) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float4 p = leaf[k];
@@ -124,7 +124,11 @@ __global__ __launch_bounds__(256) void k_refit(const float4* __restrict__ leaf, 
     box[2 * node] = make_float4(p.x, p.y, p.z, 0.f);
     box[2 * node + 1] = make_float4(p.x, p.y, p.z, 0.f);
     int par = parent[node];
-    while (par >= 0) {
+    for (int depth = 0; par >= 0; ++depth) {
+        if (depth >= 2 * kStack) {  // deeper than any well-formed tree: stop and flag
+            atomicOr(overflow, 2);
+            return;
+        }
         // release our box, acquire the sibling's: the second arrival proceeds
         const int prev = __hip_atomic_fetch_add(&arrived[par], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (prev == 0) return;
@@ -157,7 +161,9 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
     int stack[kStack];
     int sp = 0;
     stack[sp++] = 0;
+    int visits = 0;
     while (sp > 0) {
+        if (++visits > n) return false;  // a well-formed tree has n - 1 internal nodes
         const int node = stack[--sp];
         const int2 c = child[node];
         const int kids[2] = {c.x, c.y};
@@ -268,6 +274,9 @@ __global__ __launch_bounds__(256) void k_lbvh_fill(const float4* __restrict__ le
 
 }  // namespace
 
+// The overflow flag lives after the n arrival counters in ctx->lbvh.flags.
+static int* overflow_flag(lj_context* ctx, int64_t n) { return ctx->lbvh.flags.as<int>() + n + 1; }
+
 // Builds the tree over n particles into ctx->lbvh.  h is the query half-width.
 static int lbvh_build(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const lj_params* p,
                       hipStream_t s) {
@@ -298,12 +307,14 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     int* parent = L.parent.as<int>();
     int* arrived = L.flags.as<int>();
     hipLaunchKernelGGL(k_fill_i32, dim3(div_up(2 * n, 256)), dim3(256), 0, s, parent, (uint32_t)(2 * n), -1);
-    hipLaunchKernelGGL(k_fill_i32, dim3(div_up(n, 256)), dim3(256), 0, s, arrived, (uint32_t)n + 1, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(div_up(n + 1, 256)), dim3(256), 0, s, arrived, (uint32_t)n + 1, 0);
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, s, code, (int)n, L.node_child.as<int2>(),
                            parent);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, overflow_flag(ctx, n), 1u, 0);
     hipLaunchKernelGGL(k_refit, dim3(nb), dim3(256), 0, s, (const float4*)L.leaf_pos.as<float4>(), (int)n,
-                       (const int2*)L.node_child.as<int2>(), (const int*)parent, L.node_box.as<float4>(), arrived);
+                       (const int2*)L.node_child.as<int2>(), (const int*)parent, L.node_box.as<float4>(), arrived,
+                       overflow_flag(ctx, n));
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }
@@ -315,9 +326,6 @@ static int check_lbvh_params(lj_context* ctx, const lj_params* p) {
         return fail(ctx, LJ_ERR_INVALID_ARG, "the CUDALJ predicate has no cutoff; the LBVH needs one");
     return LJ_OK;
 }
-
-// The overflow flag lives after the n arrival counters in ctx->lbvh.flags.
-static int* overflow_flag(lj_context* ctx, int64_t n) { return ctx->lbvh.flags.as<int>() + n + 1; }
 
 }  // namespace lj
 
@@ -350,7 +358,6 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     LJ_HIP(ctx, ctx->bin.partials.grow((size_t)nb * 2 * sizeof(double)));
     double* partials = ctx->bin.partials.as<double>();
     int* ovf = overflow_flag(ctx, n);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ovf, 1u, 0);
     const float h = query_half_width(p);
     const float4* leaf = ctx->lbvh.leaf_pos.as<float4>();
     const int2* child = ctx->lbvh.node_child.as<int2>();
@@ -401,7 +408,6 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     LJ_HIP(ctx, ctx->bin.rows.grow(((size_t)n + 1) * sizeof(uint32_t)));
     uint32_t* rowcnt = ctx->bin.rows.as<uint32_t>();
     int* ovf = overflow_flag(ctx, n);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ovf, 1u, 0);
     const float h = query_half_width(p);
     const float4* leaf = ctx->lbvh.leaf_pos.as<float4>();
     const int2* child = ctx->lbvh.node_child.as<int2>();
@@ -419,7 +425,7 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     LJ_HIP(ctx, hipMemcpyAsync(&total, row_offsets + n, sizeof total, hipMemcpyDeviceToHost, s));
     LJ_HIP(ctx, hipMemcpyAsync(&overflow, ovf, sizeof overflow, hipMemcpyDeviceToHost, s));
     LJ_HIP(ctx, hipStreamSynchronize(s));
-    if (overflow) return fail(ctx, LJ_ERR_OVERFLOW, "LBVH deeper than the traversal stack");
+    if (overflow) return fail(ctx, LJ_ERR_OVERFLOW, "LBVH deeper than the traversal stack (flag %d)", overflow);
     if (needed) *needed = total;
     if (total > capacity) return fail(ctx, LJ_ERR_CAPACITY, "pair list needs %llu entries, capacity is %llu", total, capacity);
     if (total == 0) return LJ_OK;

@@ -53,7 +53,10 @@ EXPORTS = (
     "lj_lbvh_compute",
     "lj_lbvh_pairs",
     "lj_compute_batched",
+    "lj_profile_enable",
+    "lj_profile_read",
 )
+PHASES = ("bin", "force", "other")
 
 
 class LJParams(ctypes.Structure):
@@ -130,6 +133,8 @@ def load(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
         "lj_pairs": ([vp, vp, vp, vp, i64, P(LJParams), vp, vp, ctypes.c_ulonglong, P(ctypes.c_ulonglong), vp], ctypes.c_int),
         "lj_lbvh_pairs": ([vp, vp, vp, vp, i64, P(LJParams), vp, vp, ctypes.c_ulonglong, P(ctypes.c_ulonglong), vp], ctypes.c_int),
         "lj_compute_batched": ([vp, P(LJSystem), ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+        "lj_profile_enable": ([vp, ctypes.c_int], ctypes.c_int),
+        "lj_profile_read": ([vp, P(ctypes.c_double), P(ctypes.c_ulonglong)], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         if hasattr(lib, name):  # tests check that every declared symbol is exported
@@ -327,6 +332,17 @@ class Engine:
             fn(self.ctx, _ptr(x), _ptr(y), _ptr(z), n, ctypes.byref(p), _ptr(rows), _ptr(cols), capacity, ctypes.byref(needed), self._stream())
         )
         return rows, cols[: needed.value]
+
+    def profile(self, enable: bool = True):
+        """Turns per-phase hipEvent timing on (and resets it) or off."""
+        self._check(self.lib.lj_profile_enable(self.ctx, 1 if enable else 0))
+
+    def profile_read(self) -> dict:
+        """Summed device milliseconds and launch counts per phase since profile(True)."""
+        ms = (ctypes.c_double * 3)()
+        calls = (ctypes.c_ulonglong * 3)()
+        self._check(self.lib.lj_profile_read(self.ctx, ms, calls))
+        return {name: (ms[q], calls[q]) for q, name in enumerate(PHASES)}
 
     def compute_batched(self, systems: Sequence[LJSystem], x, y, z, energies: bool = True, out=None):
         n = x.numel()

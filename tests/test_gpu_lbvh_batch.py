@@ -1,0 +1,145 @@
+"""GPU parity of the LBVH path and of the batched path.
+
+LBVH: neighbour lists identical (bit-exact CSR) to the cell grid and the O(N^2)
+oracle; forces within 1e-5 of FP64.  Batched: every system identical to its own
+single-system call (same sorted order and sums, so the same bits) and to the
+oracle's counts and FP64 totals; a captured graph replays to the eager result.
+"""
+import numpy as np
+import pytest
+
+import ljamd
+import pyoracle as po
+from ljamd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _lbvh_case(name):
+    if name == "uniform_100k":
+        x, y, z = ljamd.generate("uniform", 1234, 100_000)
+        return (x, y, z), ljamd.make_params(0.04)
+    if name == "clusters_256k":
+        x, y, z = ljamd.generate("clusters", 1238, 262_144, [16, 0.02, 0.1, 0.9])
+        return (x, y, z), ljamd.make_params(0.005)
+    if name == "surface_100k":
+        x, y, z = ljamd.generate("surface", 1235, 100_000)
+        return (x, y, z), ljamd.make_params(0.02, box_lo=(-0.5,) * 3, box_hi=(1.5,) * 3)
+    if name == "duplicates":
+        # many identical Morton codes and identical positions (d2 = 0 pairs rejected)
+        g = np.array([0.2, 0.2, 0.2001, 0.5, 0.5, 0.5, 0.90], np.float32)
+        xs, ys, zs = np.meshgrid(g, g, g, indexing="ij")
+        return (xs.ravel(), ys.ravel(), zs.ravel()), ljamd.make_params(0.05)
+    if name == "two":
+        return (np.array([0.1, 0.12], np.float32), np.array([0.5, 0.5], np.float32), np.array([0.5, 0.5], np.float32)), ljamd.make_params(0.05)
+    if name == "single":
+        return (np.array([0.3], np.float32),) * 3, ljamd.make_params(0.05)
+    raise KeyError(name)
+
+
+LBVH_CASES = ["uniform_100k", "clusters_256k", "surface_100k", "duplicates", "two", "single"]
+
+
+@pytest.mark.parametrize("name", LBVH_CASES)
+def test_lbvh_pairs_equal_cellgrid_and_oracle(engine, name):
+    (x, y, z), p = _lbvh_case(name)
+    print(f"[lbvh pairs] {name} n={x.size}", flush=True)
+    dx, dy, dz = engine.to_device(x, y, z)
+    rows_l, cols_l = engine.pairs(dx, dy, dz, p, lbvh=True)
+    rows_c, cols_c = engine.pairs(dx, dy, dz, p)
+    engine.torch.cuda.synchronize()
+    assert np.array_equal(rows_l.cpu().numpy(), rows_c.cpu().numpy())
+    assert np.array_equal(cols_l.cpu().numpy(), cols_c.cpu().numpy())
+    rows_o, cols_o = po.pairs_cellgrid(po.from_lj(p), x, y, z)
+    assert np.array_equal(rows_l.cpu().numpy().astype(np.uint64), rows_o)
+    assert np.array_equal(cols_l.cpu().numpy().view(np.uint32), cols_o)
+    if x.size <= 4000:
+        rows_b, cols_b = po.pairs_brute(po.from_lj(p), x, y, z)
+        assert np.array_equal(rows_b, rows_o) and np.array_equal(cols_b, cols_o)
+
+
+@pytest.mark.parametrize("name", ["uniform_100k", "clusters_256k", "surface_100k", "duplicates"])
+def test_lbvh_forces_vs_fp64(engine, name):
+    (x, y, z), p = _lbvh_case(name)
+    print(f"[lbvh force] {name} n={x.size}", flush=True)
+    dx, dy, dz = engine.to_device(x, y, z)
+    out = engine.compute(dx, dy, dz, p, lbvh=True)
+    engine.torch.cuda.synchronize()
+    res = ljamd.read_results(out["result"])[0]
+    ref = po.forces_f64(po.from_lj(p), x, y, z)
+    assert int(res["n_pairs"]) == ref["n_pairs"]
+    f = [out[k][: x.size].cpu().numpy() for k in ("fx", "fy", "fz")]
+    fin = (np.abs(ref["fx"]) < 3e38) & (np.abs(ref["fy"]) < 3e38) & (np.abs(ref["fz"]) < 3e38)
+    err = po.rel_errors([a[fin] for a in f], (ref["fx"][fin], ref["fy"][fin], ref["fz"][fin]))["rel"]
+    assert err.max() <= 1e-5
+    if np.all(fin):
+        assert res["e_total"] == pytest.approx(ref["e_total"], rel=1e-9)
+
+
+def test_lbvh_rejects_cudalj(engine):
+    (x, y, z), p = _lbvh_case("two")
+    p.predicate = ljamd.PRED["cudalj"]
+    dx, dy, dz = engine.to_device(x, y, z)
+    with pytest.raises(ljamd.LJError):
+        engine.compute(dx, dy, dz, p, lbvh=True)
+
+
+def _small_batch(nsys=48):
+    return workloads.batch_c4(nsys=nsys, nmin=64, nmax=4096, seed=77)
+
+
+def test_batched_equals_single_calls_and_oracle(engine):
+    systems, X, Y, Z = _small_batch()
+    dx, dy, dz = engine.to_device(X, Y, Z)
+    out = engine.compute_batched(systems, dx, dy, dz)
+    engine.torch.cuda.synchronize()
+    res = ljamd.read_results(out["result"], len(systems))
+    fb = [out[k][: X.size].cpu().numpy() for k in ("fx", "fy", "fz", "e")]
+    for s, sy in enumerate(systems):
+        a, b = sy.first, sy.first + sy.n
+        single = engine.compute(dx[a:b], dy[a:b], dz[a:b], sy.params)
+        engine.torch.cuda.synchronize()
+        r1 = ljamd.read_results(single["result"])[0]
+        for k, arr in zip(("fx", "fy", "fz", "e"), fb):
+            assert np.array_equal(single[k][: sy.n].cpu().numpy().view(np.uint32), arr[a:b].view(np.uint32)), (s, k)
+        assert int(r1["n_pairs"]) == int(res[s]["n_pairs"])
+        assert int(r1["n_candidates"]) == int(res[s]["n_candidates"])
+        ref = po.forces_f64(po.from_lj(sy.params), X[a:b], Y[a:b], Z[a:b])
+        assert int(res[s]["n_pairs"]) == ref["n_pairs"]
+        assert int(res[s]["n_candidates"]) == ref["n_candidates"]
+        assert res[s]["e_total"] == pytest.approx(ref["e_total"], rel=1e-9)
+        assert res[s]["virial"] == pytest.approx(ref["virial"], rel=1e-9)
+
+
+def test_batched_graph_replays_match_eager(engine):
+    torch = engine.torch
+    systems, X, Y, Z = _small_batch(32)
+    dx, dy, dz = engine.to_device(X, Y, Z)
+    out = engine.compute_batched(systems, dx, dy, dz)
+    torch.cuda.synchronize()
+    eager = {k: out[k].clone() for k in ("fx", "fy", "fz", "e", "result")}
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        engine.compute_batched(systems, dx, dy, dz, out=out)
+    torch.cuda.current_stream().wait_stream(s)
+    with torch.cuda.graph(g):
+        engine.compute_batched(systems, dx, dy, dz, out=out)
+    for rep in range(3):
+        for k in ("fx", "fy", "fz", "e", "result"):
+            out[k].zero_()
+        engine.compute(dx[:500], dy[:500], dz[:500], ljamd.make_params(0.1))  # unrelated eager work
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k in ("fx", "fy", "fz", "e", "result"):
+            assert torch.equal(out[k], eager[k]), (rep, k)
+
+
+def test_batched_rejects_unpacked_systems(engine):
+    systems, X, Y, Z = _small_batch(4)
+    systems[2].first += 1
+    dx, dy, dz = engine.to_device(X, Y, Z)
+    with pytest.raises(ljamd.LJError):
+        engine.compute_batched(systems, dx, dy, dz)
