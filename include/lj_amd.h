@@ -65,8 +65,9 @@ extern "C" {
 #define LJ_PREC_FP32_COMP 2
 
 /* lj_params.flags */
-#define LJ_FLAG_NO_VIRIAL 1u     /* skip the virial sum */
-#define LJ_FLAG_KERNEL_SIMPLE 2u /* use the one-thread-per-particle force kernel */
+#define LJ_FLAG_NO_VIRIAL 1u /* skip the virial sum */
+
+
 
 /* Generators for lj_generate. */
 #define LJ_GEN_UNIFORM 0  /* gparams {lo, hi}: i.i.d. U[lo,hi)^3 */
@@ -87,6 +88,9 @@ typedef struct lj_params {
     int32_t predicate; /* LJ_PRED_* */
     int32_t precision; /* LJ_PREC_* */
     uint32_t flags;    /* LJ_FLAG_* */
+    int32_t refine;    /* force-grid cells per cell_edge along each axis: 0 = choose, 1, 2 or 4;
+                          the candidate count keeps the cell_edge grid of the reference */
+    int32_t reserved;  /* must be 0 */
 } lj_params;
 
 /* Scalar results, written to DEVICE memory by the compute calls. */

@@ -74,6 +74,9 @@ int make_geom(lj_context* ctx, const lj_params* p, int64_t n, GridGeom* g, PairC
     c->softd = (double)c->soft;
     if (p->precision < LJ_PREC_MIXED || p->precision > LJ_PREC_FP32_COMP)
         return fail(ctx, LJ_ERR_INVALID_ARG, "unknown precision %d", p->precision);
+    if (p->refine != 0 && p->refine != 1 && p->refine != 2 && p->refine != 4)
+        return fail(ctx, LJ_ERR_INVALID_ARG, "refine must be 0, 1, 2 or 4 (got %d)", p->refine);
+    if (p->reserved != 0) return fail(ctx, LJ_ERR_INVALID_ARG, "lj_params.reserved must be 0");
     c->tri_lo = tri_lo();
     c->e4 = 4.0 * (double)p->epsilon;
     c->e24 = 24.0 * (double)p->epsilon;
@@ -167,7 +170,7 @@ static void release_all(lj_context* ctx) {
         }
     BinWork& b = ctx->bin;
     for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.vals[0], &b.vals[1], &b.hist, &b.scan_tmp, &b.counts, &b.offsets,
-                      &b.pos, &b.cell_of, &b.partials, &b.counters, &b.rows})
+                      &b.pos, &b.cell_of, &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets})
         d->release();
     LbvhWork& l = ctx->lbvh;
     for (DevBuf* d : {&l.morton, &l.order[0], &l.order[1], &l.node_box, &l.node_child, &l.parent, &l.flags, &l.leaf_pos})

@@ -369,12 +369,16 @@ int gather_sorted(lj_context* ctx, const float* x, const float* y, const float* 
     return LJ_OK;
 }
 
-int count_candidates(lj_context* ctx, const GridGeom& g, unsigned long long* out, hipStream_t s) {
+int count_candidates_from(lj_context* ctx, const GridGeom& g, const uint32_t* offsets, unsigned long long* out,
+                          hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, out, 1u, 0ull);
-    hipLaunchKernelGGL(k_candidates, dim3(div_up(g.ncells, 256)), dim3(256), 0, s,
-                       ctx->bin.offsets.as<uint32_t>(), g, out);
+    hipLaunchKernelGGL(k_candidates, dim3(div_up(g.ncells, 256)), dim3(256), 0, s, offsets, g, out);
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
+}
+
+int count_candidates(lj_context* ctx, const GridGeom& g, unsigned long long* out, hipStream_t s) {
+    return count_candidates_from(ctx, g, ctx->bin.offsets.as<uint32_t>(), out, s);
 }
 
 void fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s) {

@@ -61,6 +61,8 @@ struct BinWork {
     DevBuf partials;     // per-block partial sums (double) for deterministic totals
     DevBuf counters;     // u64 counters
     DevBuf rows;         // u64 pair-list row offsets
+    DevBuf ccounts;      // u32 counts of the cell_edge grid when the force grid is refined
+    DevBuf coffsets;     // u32 offsets of the cell_edge grid (ncells+1)
     int cur = 0;         // which keys/vals buffer holds the sorted result
     const uint32_t* sorted_keys() const { return keys[cur].as<uint32_t>(); }
     const uint32_t* perm() const { return vals[cur].as<uint32_t>(); }
@@ -163,6 +165,9 @@ int bin_particles(lj_context* ctx, const float* x, const float* y, const float* 
 // Candidate count of the 27-cell stencil from ctx->bin.offsets, into *out (device u64),
 // which is overwritten (not accumulated).
 int count_candidates(lj_context* ctx, const GridGeom& g, unsigned long long* out, hipStream_t s);
+// Same from explicit offsets of grid g.
+int count_candidates_from(lj_context* ctx, const GridGeom& g, const uint32_t* offsets, unsigned long long* out,
+                          hipStream_t s);
 
 void fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);

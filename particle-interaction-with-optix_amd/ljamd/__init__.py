@@ -35,7 +35,7 @@ ERRORS = {
 PRED = {"aabb": 0, "custom": 1, "cudalj": 2, "sphere": 3, "triangle": 4}
 PREC = {"mixed": 0, "fp32": 1, "fp32_comp": 2}
 FLAG_NO_VIRIAL = 1
-FLAG_KERNEL_SIMPLE = 2
+
 GEN = {"uniform": 0, "surface": 1, "clusters": 2}
 
 # Every symbol include/lj_amd.h declares.
@@ -71,6 +71,8 @@ class LJParams(ctypes.Structure):
         ("predicate", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("flags", ctypes.c_uint32),
+        ("refine", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -157,6 +159,7 @@ def make_params(
     predicate: str | int = "aabb",
     precision: str | int = "mixed",
     flags: int = 0,
+    refine: int = 0,
 ) -> LJParams:
     """Run parameters; sigma defaults to rc / 2.5 (the workloads' choice)."""
     p = LJParams()
@@ -171,6 +174,8 @@ def make_params(
     p.predicate = PRED[predicate] if isinstance(predicate, str) else int(predicate)
     p.precision = PREC[precision] if isinstance(precision, str) else int(precision)
     p.flags = flags
+    p.refine = refine
+    p.reserved = 0
     return p
 
 
@@ -186,6 +191,7 @@ def params_to_dict(p: LJParams) -> dict:
         "predicate": p.predicate,
         "precision": p.precision,
         "flags": p.flags,
+        "refine": p.refine,
     }
 
 

@@ -29,11 +29,13 @@ def test_header_declarations_are_all_exported():
 
 
 def test_struct_layouts_match_header_sizes():
-    # lj_params: 4 floats, 2x3 floats, 1 float, 3 x 32-bit ints = 56 bytes
-    assert ctypes.sizeof(ljamd.LJParams) == 56
+    # lj_params: 4 floats, 2x3 floats, 1 float, 5 x 32-bit ints = 64 bytes
+    assert ctypes.sizeof(ljamd.LJParams) == 64
     assert ctypes.sizeof(ljamd.LJResult) == 32
-    assert ctypes.sizeof(ljamd.LJSystem) == 16 + 56
-    assert ctypes.sizeof(po.OrcParams) == ctypes.sizeof(ljamd.LJParams)
+    assert ctypes.sizeof(ljamd.LJSystem) == 16 + 64
+    # the oracle's params are the leading fields of the engine's (same offsets)
+    for name, _ in po.OrcParams._fields_:
+        assert getattr(po.OrcParams, name).offset == getattr(ljamd.LJParams, name).offset
 
 
 def test_version_string():

@@ -103,6 +103,40 @@ def test_forces_mixed_vs_fp64(engine, name):
         assert res["virial"] == pytest.approx(ref["virial"], rel=1e-9, abs=1e-12)
 
 
+@pytest.mark.parametrize("refine", [1, 2, 4])
+@pytest.mark.parametrize("name", ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "edges", "one_cell_crowd", "tiny_7"])
+def test_refined_grids_same_pairs_and_forces(engine, name, refine):
+    """The 27-cell grid and the half/quarter-edge grids with stencil trimming visit
+    every accepted pair: counts bit-exact, forces within the bar."""
+    (x, y, z), p = _case(name)
+    p.refine = refine
+    print(f"[refine] {name} k={refine}", flush=True)
+    ref = po.forces_f64(po.from_lj(p), x, y, z)
+    f, e, res = _forces(engine, x, y, z, p)
+    assert int(res["n_pairs"]) == ref["n_pairs"]
+    assert int(res["n_candidates"]) == ref["n_candidates"]
+    fin = (np.abs(ref["fx"]) < 3e38) & (np.abs(ref["fy"]) < 3e38) & (np.abs(ref["fz"]) < 3e38)
+    err = po.rel_errors([a[fin] for a in f], (ref["fx"][fin], ref["fy"][fin], ref["fz"][fin]))["rel"]
+    assert err.max() <= FORCE_TOL
+
+
+def test_trimming_at_cell_faces(engine):
+    """Pairs at exactly the cutoff across cell faces and corners, for every refinement."""
+    rc = np.float32(0.0625)  # cell faces at exact binary fractions
+    base = np.array([0.25, 0.25, 0.25], np.float32)
+    pts = [base]
+    for dxyz in ([rc, 0, 0], [0, rc, 0], [0, 0, -rc], [rc / 2, rc / 2, 0], [-rc * 0.57735, rc * 0.57735, rc * 0.57735],
+                 [np.float32(0.0625 * 0.99999), 0, 0], [np.float32(0.0625 * 1.00001), 0, 0]):
+        pts.append(base + np.array(dxyz, np.float32))
+    pts = np.array(pts, np.float32)
+    x, y, z = (np.ascontiguousarray(pts[:, d]) for d in range(3))
+    for refine in (1, 2, 4):
+        p = ljamd.make_params(float(rc), refine=refine)
+        rows, cols = po.pairs_brute(po.from_lj(p), x, y, z)
+        f, e, res = _forces(engine, x, y, z, p)
+        assert int(res["n_pairs"]) == int(rows[-1]), refine
+
+
 @pytest.mark.parametrize("prec,tol", [("fp32", 1e-4), ("fp32_comp", 1e-4)])
 def test_forces_fp32_modes(engine, prec, tol):
     (x, y, z), p = _case("c1_uniform_100k")
@@ -182,6 +216,7 @@ def test_invalid_params_rejected(engine):
         ljamd.make_params(0.1, box_lo=(1, 0, 0), box_hi=(0, 1, 1)),
         ljamd.make_params(0.1, predicate=9),
         ljamd.make_params(1e-9),
+        ljamd.make_params(0.1, refine=3),
     ):
         with pytest.raises(ljamd.LJError):
             engine.compute(dx, dy, dz, bad)
