@@ -169,11 +169,11 @@ static void release_all(lj_context* ctx) {
             (void)hipEventDestroy(e.second);
         }
     BinWork& b = ctx->bin;
-    for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.vals[0], &b.vals[1], &b.hist, &b.scan_tmp, &b.counts, &b.offsets,
-                      &b.pos, &b.cell_of, &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets})
+    for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.hist, &b.scan_tmp, &b.counts, &b.offsets, &b.pos, &b.pos_alt,
+                      &b.cell_of, &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets})
         d->release();
     LbvhWork& l = ctx->lbvh;
-    for (DevBuf* d : {&l.morton, &l.order[0], &l.order[1], &l.node_box, &l.node_child, &l.parent, &l.flags, &l.leaf_pos})
+    for (DevBuf* d : {&l.node_box, &l.node_child, &l.parent, &l.flags})
         d->release();
     BatchWork& t = ctx->batch;
     for (DevBuf* d : {&t.sys, &t.cell_base, &t.part_sys, &t.tiles, &t.partials, &t.results_tmp}) d->release();
@@ -216,8 +216,7 @@ int lj_bin(lj_context* ctx, const float* x, const float* y, const float* z, int6
     if (cell_offsets)
         LJ_HIP(ctx, hipMemcpyAsync(cell_offsets, ctx->bin.offsets.ptr, ((size_t)g.ncells + 1) * 4,
                                    hipMemcpyDeviceToDevice, s));
-    if (perm && n > 0)
-        LJ_HIP(ctx, hipMemcpyAsync(perm, ctx->bin.perm(), (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    if (perm) LJ_TRY(extract_perm(ctx, n, perm, s));
     if (n_candidates) LJ_TRY(count_candidates(ctx, g, n_candidates, s));
     return LJ_OK;
 }

@@ -13,6 +13,7 @@
 
 #include "lj_internal.h"
 #include "lj_pair.h"
+#include "lj_sort.h"
 
 namespace lj {
 
@@ -35,20 +36,17 @@ __device__ inline uint32_t find_sys(const uint32_t* __restrict__ base, uint32_t 
     return lo;
 }
 
-__global__ __launch_bounds__(256) void k_batch_keys(const float* __restrict__ x, const float* __restrict__ y,
-                                                    const float* __restrict__ z, uint32_t n,
-                                                    const SysDesc* __restrict__ sys, const uint32_t* __restrict__ first,
-                                                    uint32_t nsys, uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals, uint32_t* __restrict__ counts) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = find_sys(first, nsys, i);
-    const SysDesc& d = sys[s];
-    const uint32_t k = d.cell_base + cell_key(d.g, x[i], y[i], z[i]);
-    keys[i] = k;
-    vals[i] = i;
-    atomicAdd(&counts[k], 1u);
-}
+// Global cell key of particle i: its system's first cell plus the cell in that
+// system's grid.
+struct BatchKeyOp {
+    const SysDesc* sys;
+    const uint32_t* first;
+    uint32_t nsys;
+    __device__ uint32_t operator()(float x, float y, float z, uint32_t i) const {
+        const SysDesc& d = sys[find_sys(first, nsys, i)];
+        return d.cell_base + cell_key(d.g, x, y, z);
+    }
+};
 
 __global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __restrict__ off, uint32_t ncells,
                                                           const SysDesc* __restrict__ sys,
@@ -217,30 +215,18 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     const int64_t n = (int64_t)ntot;
     const uint32_t ncells = (uint32_t)ctot;
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    LJ_HIP(ctx, B.keys[0].grow(nn * 4));
-    LJ_HIP(ctx, B.keys[1].grow(nn * 4));
-    LJ_HIP(ctx, B.vals[0].grow(nn * 4));
-    LJ_HIP(ctx, B.vals[1].grow(nn * 4));
-    LJ_HIP(ctx, B.counts.grow((size_t)ncells * 4));
-    LJ_HIP(ctx, B.offsets.grow(((size_t)ncells + 1) * 4));
-    LJ_HIP(ctx, B.pos.grow(nn * sizeof(float4)));
     LJ_HIP(ctx, W.partials.grow(nn * sizeof(double2)));
     LJ_HIP(ctx, W.part_sys.grow(nn * sizeof(uint32_t)));
-    uint32_t* counts = B.counts.as<uint32_t>();
     const unsigned nb = div_up(n > 0 ? n : 1, 256);
-    uint32_t* off = B.offsets.as<uint32_t>();
+    uint32_t* off;
     {
-    PhaseTimer tb(ctx, LJ_PHASE_BIN, s);
-    fill_u32(counts, ncells, 0u, s);
-    fill_u64(reinterpret_cast<unsigned long long*>(results), (uint32_t)(4 * nsys), 0ull, s);
-    if (n > 0)
-        hipLaunchKernelGGL(k_batch_keys, dim3(nb), dim3(256), 0, s, x, y, z, (uint32_t)n, dsys, dfirst, (uint32_t)nsys,
-                           B.keys[0].as<uint32_t>(), B.vals[0].as<uint32_t>(), counts);
-    LJ_TRY(scan_u32(ctx, counts, off, ncells, off + ncells, s));
-    int bits = 0;
-    while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)ncells) ++bits;
-    LJ_TRY(radix_sort(ctx, n, bits, s));
-    LJ_TRY(gather_sorted(ctx, x, y, z, n, s));
+        PhaseTimer tb(ctx, LJ_PHASE_BIN, s);
+        fill_u64(reinterpret_cast<unsigned long long*>(results), (uint32_t)(4 * nsys), 0ull, s);
+        int bits = 0;
+        while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)ncells) ++bits;
+        LJ_TRY(sort_particles(ctx, x, y, z, n, BatchKeyOp{dsys, dfirst, (uint32_t)nsys}, bits, s));
+        LJ_TRY(cell_offsets_from_sorted(ctx, n, ncells, s));
+        off = B.offsets.as<uint32_t>();
     }
     hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(ncells, 256)), dim3(256), 0, s, (const uint32_t*)off, ncells,
                        dsys, dcbase, (uint32_t)nsys, results);

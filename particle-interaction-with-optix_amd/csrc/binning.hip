@@ -17,41 +17,18 @@
 //    atomic tickets, cudaLJ.cu:462);
 //  - the candidate count is a closed form over cells, not a pass over particles.
 #include "lj_internal.h"
+#include "lj_sort.h"
 
 namespace lj {
 
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 2048 values per block
 
 __device__ inline unsigned lane_id() { return __lane_id(); }
-
-// popcount(mask & lanes below this one)
-__device__ inline unsigned rank_below(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-// ---------------------------------------------------------------- keys
-__global__ __launch_bounds__(256) void k_keys(const float* __restrict__ x, const float* __restrict__ y,
-                                              const float* __restrict__ z, uint32_t n, GridGeom g,
-                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                              uint32_t* __restrict__ counts) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = cell_key(g, x[i], y[i], z[i]);
-    keys[i] = k;
-    vals[i] = i;
-    atomicAdd(&counts[k], 1u);
-}
 
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,85 +100,37 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI* in, TS* ou
     if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) *total = run;
 }
 
-// ---------------------------------------------------------------- radix sort
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                            int shift, uint32_t ntiles,
-                                                            uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t base = blockIdx.x * kSortTile;
-#pragma unroll 4
-    for (int r = 0; r < kSortItems; ++r) {
-        const uint32_t idx = base + r * kSortThreads + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadix - 1)], 1u);
-    }
-    __syncthreads();
-    hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// Stable scatter of one tile.  Keys are taken in rounds of 256 in input order; inside
-// a round, waves come in order and lanes in order, so (round, wave, lane) is the
-// input order.  Lanes holding the same digit are found with 8 ballots (one per
-// digit bit); a lane's rank among them is a masked popcount.
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
-    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t n, int shift,
-    uint32_t ntiles, const uint32_t* __restrict__ hist, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout) {
-    constexpr int kWaves = kSortThreads / kWave;
-    __shared__ uint32_t gbase[kRadix];
-    __shared__ uint32_t wcnt[kWaves][kRadix];
-    const unsigned tid = threadIdx.x;
-    const unsigned wave = tid / kWave;
-    gbase[tid] = hist[(uint64_t)tid * ntiles + blockIdx.x];
-    const uint32_t base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortItems; ++r) {
-        if (base + r * kSortThreads >= n) break;  // uniform across the block
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
-        __syncthreads();
-        const uint32_t idx = base + r * kSortThreads + tid;
-        const bool valid = idx < n;
-        const uint32_t key = valid ? kin[idx] : 0u;
-        const uint32_t d = (key >> shift) & (kRadix - 1);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < kRadixBits; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bb = __ballot(bit);
-            peers &= bit ? bb : ~bb;
-        }
-        const unsigned rank = rank_below(peers);
-        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            uint32_t run = gbase[tid];
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = wcnt[w][tid];
-                wcnt[w][tid] = run;
-                run += c;
-            }
-            gbase[tid] = run;
-        }
-        __syncthreads();
-        if (valid) {
-            const uint32_t dst = wcnt[wave][d] + rank;
-            kout[dst] = key;
-            vout[dst] = vin[idx];
-        }
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------- gather
-__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ x, const float* __restrict__ y,
-                                                const float* __restrict__ z, const uint32_t* __restrict__ perm,
-                                                uint32_t n, float4* __restrict__ pos) {
+// ---------------------------------------------------------------- cell counts
+// Number of particles of each occupied cell from the sorted keys: the first slot of a
+// run gallops to the run's end (O(log run length)); empty cells keep the zero fill.
+__global__ __launch_bounds__(256) void k_run_counts(const uint32_t* __restrict__ skeys, uint32_t n,
+                                                    uint32_t* __restrict__ counts) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t i = perm[k];
-    pos[k] = make_float4(x[i], y[i], z[i], __uint_as_float(i));
+    const uint32_t key = skeys[k];
+    if (k > 0 && skeys[k - 1] == key) return;
+    uint32_t lo = k, hi = n, step = 1;  // skeys[lo] == key; hi == n or skeys[hi] != key
+    while (true) {
+        const uint32_t probe = lo + step;
+        if (probe >= n || skeys[probe] != key) {
+            hi = probe < n ? probe : n;
+            break;
+        }
+        lo = probe;
+        step <<= 1;
+    }
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (skeys[mid] == key) lo = mid;
+        else hi = mid;
+    }
+    counts[key] = hi - k;
+}
+
+__global__ __launch_bounds__(256) void k_perm_from_pos(const float4* __restrict__ pos, uint32_t n,
+                                                       uint32_t* __restrict__ perm) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) perm[k] = __float_as_uint(pos[k].w);
 }
 
 // ---------------------------------------------------------------- candidates
@@ -313,58 +242,29 @@ int scan_u32_to_u64(lj_context* ctx, const uint32_t* in, unsigned long long* out
     return scan_any<uint32_t, unsigned long long>(ctx, in, out, n, out + (n > 0 ? n : 0), s);
 }
 
-int radix_sort(lj_context* ctx, int64_t n, int bits, hipStream_t s) {
+int cell_offsets_from_sorted(lj_context* ctx, int64_t n, uint32_t ncells, hipStream_t s) {
     BinWork& w = ctx->bin;
-    const unsigned ntiles = div_up(n, kSortTile);
-    LJ_HIP(ctx, w.hist.grow((size_t)kRadix * (ntiles ? ntiles : 1) * 4));
-    int cur = 0;
-    if (n > 1) {
-        for (int shift = 0; shift < bits; shift += kRadixBits) {
-            uint32_t* hist = w.hist.as<uint32_t>();
-            hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kSortThreads), 0, s, w.keys[cur].as<uint32_t>(),
-                               (uint32_t)n, shift, ntiles, hist);
-            LJ_TRY(scan_u32(ctx, hist, hist, (int64_t)kRadix * ntiles, nullptr, s));
-            hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, s, w.keys[cur].as<uint32_t>(),
-                               w.vals[cur].as<uint32_t>(), (uint32_t)n, shift, ntiles, hist,
-                               w.keys[cur ^ 1].as<uint32_t>(), w.vals[cur ^ 1].as<uint32_t>());
-            cur ^= 1;
-        }
-    }
-    w.cur = cur;
-    LJ_HIP(ctx, hipGetLastError());
-    return LJ_OK;
+    LJ_HIP(ctx, w.counts.grow((size_t)ncells * 4));
+    LJ_HIP(ctx, w.offsets.grow(((size_t)ncells + 1) * 4));
+    uint32_t* counts = w.counts.as<uint32_t>();
+    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(ncells, 256)), dim3(256), 0, s, counts, (uint64_t)ncells, 0u);
+    if (n > 0)
+        hipLaunchKernelGGL(k_run_counts, dim3(div_up(n, 256)), dim3(256), 0, s, w.sorted_keys(), (uint32_t)n, counts);
+    // offsets[0..nc) = exclusive scan of counts, offsets[nc] = n
+    uint32_t* off = w.offsets.as<uint32_t>();
+    return scan_u32(ctx, counts, off, ncells, off + ncells, s);
 }
 
 int bin_particles(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
                   const GridGeom& g, hipStream_t s) {
-    BinWork& w = ctx->bin;
-    const size_t nn = (size_t)(n > 0 ? n : 1);
-    LJ_HIP(ctx, w.keys[0].grow(nn * 4));
-    LJ_HIP(ctx, w.keys[1].grow(nn * 4));
-    LJ_HIP(ctx, w.vals[0].grow(nn * 4));
-    LJ_HIP(ctx, w.vals[1].grow(nn * 4));
-    LJ_HIP(ctx, w.counts.grow((size_t)g.ncells * 4));
-    LJ_HIP(ctx, w.offsets.grow(((size_t)g.ncells + 1) * 4));
-    LJ_HIP(ctx, w.pos.grow(nn * sizeof(float4)));
-    uint32_t* counts = w.counts.as<uint32_t>();
-    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(g.ncells, 256)), dim3(256), 0, s, counts,
-                       (uint64_t)g.ncells, 0u);
-    if (n > 0)
-        hipLaunchKernelGGL(k_keys, dim3(div_up(n, 256)), dim3(256), 0, s, x, y, z, (uint32_t)n, g,
-                           w.keys[0].as<uint32_t>(), w.vals[0].as<uint32_t>(), counts);
-    // offsets[0..nc) = exclusive scan of counts, offsets[nc] = n
-    uint32_t* off = w.offsets.as<uint32_t>();
-    LJ_TRY(scan_u32(ctx, counts, off, g.ncells, off + g.ncells, s));
-
-    LJ_TRY(radix_sort(ctx, n, key_bits(g.ncells), s));
-    return gather_sorted(ctx, x, y, z, n, s);
+    LJ_TRY(sort_particles(ctx, x, y, z, n, CellKeyOp{g}, key_bits(g.ncells), s));
+    return cell_offsets_from_sorted(ctx, n, g.ncells, s);
 }
 
-int gather_sorted(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, hipStream_t s) {
-    BinWork& w = ctx->bin;
+int extract_perm(lj_context* ctx, int64_t n, uint32_t* perm, hipStream_t s) {
     if (n > 0)
-        hipLaunchKernelGGL(k_gather, dim3(div_up(n, 256)), dim3(256), 0, s, x, y, z, w.perm(), (uint32_t)n,
-                           w.pos.as<float4>());
+        hipLaunchKernelGGL(k_perm_from_pos, dim3(div_up(n, 256)), dim3(256), 0, s, ctx->bin.pos.as<float4>(),
+                           (uint32_t)n, perm);
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }

@@ -143,18 +143,27 @@ __global__ __launch_bounds__(256) void k_coarse_counts(const uint32_t* __restric
     ccount[c] = s;
 }
 
-// Final FP64 reduction of the per-block partials, in a fixed order.
+// FP64 reduction of the per-block partials in a fixed order: stage 1 sums chunks of
+// 1024 pairs into out (one block per chunk), stage 2 (one block, res != NULL) sums
+// the chunk results into res.
 __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ partials, uint32_t nblocks,
-                                                  lj_result* __restrict__ res) {
+                                                  double* __restrict__ out, lj_result* __restrict__ res) {
     double a = 0.0, b = 0.0;
-    for (uint32_t q = threadIdx.x; q < nblocks; q += 256) {
+    const uint32_t lo = res ? 0 : blockIdx.x * 1024u;
+    const uint32_t hi = res ? nblocks : (lo + 1024u < nblocks ? lo + 1024u : nblocks);
+    for (uint32_t q = lo + threadIdx.x; q < hi; q += 256) {
         a += partials[2 * q];
         b += partials[2 * q + 1];
     }
     block_sum2(a, b);
     if (threadIdx.x == 0) {
-        res->e_total = 0.5 * a;
-        res->virial = 0.5 * b;
+        if (res) {
+            res->e_total = 0.5 * a;
+            res->virial = 0.5 * b;
+        } else {
+            out[2 * blockIdx.x] = a;
+            out[2 * blockIdx.x + 1] = b;
+        }
     }
 }
 
@@ -252,8 +261,19 @@ void launch_force(int prec, bool vir, dim3 grid, hipStream_t s, const float4* po
 
 }  // namespace
 
-void finalize_energy(const double* partials, unsigned nblocks, lj_result* res, hipStream_t s) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, (uint32_t)nblocks, res);
+void finalize_energy(double* partials, unsigned nblocks, lj_result* res, hipStream_t s) {
+    if (nblocks > 1024) {
+        // stage 1 writes behind the nblocks input pairs (the partials buffer holds room)
+        const unsigned chunks = (nblocks + 1023) / 1024;
+        double* tmp = partials + 2 * (size_t)nblocks;
+        hipLaunchKernelGGL(k_finalize, dim3(chunks), dim3(256), 0, s, (const double*)partials, (uint32_t)nblocks, tmp,
+                           (lj_result*)nullptr);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, (const double*)tmp, (uint32_t)chunks, (double*)nullptr,
+                           res);
+    } else {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, (const double*)partials, (uint32_t)nblocks,
+                           (double*)nullptr, res);
+    }
 }
 
 // Scalar results live in the caller's lj_result or, when it is NULL, in a scratch one.
@@ -341,7 +361,7 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
         fill_u64(&res->n_pairs, 1, 0ull, s);
     }
     const unsigned nb = div_up(n > 0 ? n : 1, 256);
-    LJ_HIP(ctx, ctx->bin.partials.grow((size_t)nb * 2 * sizeof(double)));
+    LJ_HIP(ctx, ctx->bin.partials.grow(((size_t)nb + nb / 1024 + 1) * 2 * sizeof(double)));
     double* partials = ctx->bin.partials.as<double>();
     const int prec = p->precision;
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
@@ -456,8 +476,7 @@ int lj_reorder(lj_context* ctx, const float* x, const float* y, const float* z, 
     hipStream_t s = (hipStream_t)stream;
     LJ_TRY(bin_particles(ctx, x, y, z, n, g, s));
     LJ_TRY(unpack_sorted(ctx, n, xo, yo, zo, s));
-    if (perm && n > 0)
-        LJ_HIP(ctx, hipMemcpyAsync(perm, ctx->bin.perm(), (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    if (perm) LJ_TRY(extract_perm(ctx, n, perm, s));
     return LJ_OK;
 }
 

@@ -23,6 +23,7 @@
 // predicate can accept has |d_axis| <= rc (1 + 2^-20).
 #include "lj_internal.h"
 #include "lj_pair.h"
+#include "lj_sort.h"
 
 namespace lj {
 
@@ -49,26 +50,14 @@ __device__ inline uint32_t quant10(float x, float lo, float inv) {
     return (uint32_t)q;
 }
 
-__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ x, const float* __restrict__ y,
-                                                const float* __restrict__ z, uint32_t n, float lx, float ly, float lz,
-                                                float ix, float iy, float iz, uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    keys[i] = (spread10(quant10(x[i], lx, ix)) << 2) | (spread10(quant10(y[i], ly, iy)) << 1) |
-              spread10(quant10(z[i], lz, iz));
-    vals[i] = i;
-}
-
-// Leaf positions in Morton order; w carries the original index.
-__global__ __launch_bounds__(256) void k_leaves(const float* __restrict__ x, const float* __restrict__ y,
-                                                const float* __restrict__ z, const uint32_t* __restrict__ order,
-                                                uint32_t n, float4* __restrict__ leaf) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t i = order[k];
-    leaf[k] = make_float4(x[i], y[i], z[i], __uint_as_float(i));
-}
+// 30-bit Morton code of a position, quantised to 1024 steps per axis of the box.
+struct MortonKeyOp {
+    float lo[3], inv[3];
+    __device__ uint32_t operator()(float x, float y, float z, uint32_t) const {
+        return (spread10(quant10(x, lo[0], inv[0])) << 2) | (spread10(quant10(y, lo[1], inv[1])) << 1) |
+               spread10(quant10(z, lo[2], inv[2]));
+    }
+};
 
 // Common-prefix length of sorted keys i and j (Karras' delta); equal codes fall back
 // to the indices, which makes every key distinct.
@@ -283,27 +272,20 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     LbvhWork& L = ctx->lbvh;
     BinWork& B = ctx->bin;
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    LJ_HIP(ctx, B.keys[0].grow(nn * 4));
-    LJ_HIP(ctx, B.keys[1].grow(nn * 4));
-    LJ_HIP(ctx, B.vals[0].grow(nn * 4));
-    LJ_HIP(ctx, B.vals[1].grow(nn * 4));
-    LJ_HIP(ctx, L.leaf_pos.grow(nn * sizeof(float4)));
     LJ_HIP(ctx, L.node_child.grow(nn * sizeof(int2)));
     LJ_HIP(ctx, L.parent.grow(2 * nn * sizeof(int)));
     LJ_HIP(ctx, L.flags.grow(nn * sizeof(int) + 64));
     LJ_HIP(ctx, L.node_box.grow(2 * nn * 2 * sizeof(float4)));
-    if (n == 0) return LJ_OK;
-    float lo[3], inv[3];
+    MortonKeyOp op;
     for (int d = 0; d < 3; ++d) {
-        lo[d] = p->box_lo[d];
-        inv[d] = 1024.0f / (p->box_hi[d] - p->box_lo[d]);
+        op.lo[d] = p->box_lo[d];
+        op.inv[d] = 1024.0f / (p->box_hi[d] - p->box_lo[d]);
     }
+    // the sorted payload is the leaf array: positions in Morton order, w = original index
+    LJ_TRY(sort_particles(ctx, x, y, z, n, op, 30, s));
+    if (n == 0) return LJ_OK;
     const unsigned nb = div_up(n, 256);
-    hipLaunchKernelGGL(k_morton, dim3(nb), dim3(256), 0, s, x, y, z, (uint32_t)n, lo[0], lo[1], lo[2], inv[0], inv[1],
-                       inv[2], B.keys[0].as<uint32_t>(), B.vals[0].as<uint32_t>());
-    LJ_TRY(radix_sort(ctx, n, 30, s));
     const uint32_t* code = B.sorted_keys();
-    hipLaunchKernelGGL(k_leaves, dim3(nb), dim3(256), 0, s, x, y, z, B.perm(), (uint32_t)n, L.leaf_pos.as<float4>());
     int* parent = L.parent.as<int>();
     int* arrived = L.flags.as<int>();
     hipLaunchKernelGGL(k_fill_i32, dim3(div_up(2 * n, 256)), dim3(256), 0, s, parent, (uint32_t)(2 * n), -1);
@@ -312,7 +294,7 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
         hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, s, code, (int)n, L.node_child.as<int2>(),
                            parent);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, overflow_flag(ctx, n), 1u, 0);
-    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(256), 0, s, (const float4*)L.leaf_pos.as<float4>(), (int)n,
+    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(256), 0, s, (const float4*)B.pos.as<float4>(), (int)n,
                        (const int2*)L.node_child.as<int2>(), (const int*)parent, L.node_box.as<float4>(), arrived,
                        overflow_flag(ctx, n));
     LJ_HIP(ctx, hipGetLastError());
@@ -355,11 +337,11 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     }
     fill_u64(&res->n_candidates, 2, 0ull, s);
     const unsigned nb = div_up(n > 0 ? n : 1, 256);
-    LJ_HIP(ctx, ctx->bin.partials.grow((size_t)nb * 2 * sizeof(double)));
+    LJ_HIP(ctx, ctx->bin.partials.grow(((size_t)nb + nb / 1024 + 1) * 2 * sizeof(double)));
     double* partials = ctx->bin.partials.as<double>();
     int* ovf = overflow_flag(ctx, n);
     const float h = query_half_width(p);
-    const float4* leaf = ctx->lbvh.leaf_pos.as<float4>();
+    const float4* leaf = ctx->bin.pos.as<float4>();
     const int2* child = ctx->lbvh.node_child.as<int2>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
@@ -409,7 +391,7 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     uint32_t* rowcnt = ctx->bin.rows.as<uint32_t>();
     int* ovf = overflow_flag(ctx, n);
     const float h = query_half_width(p);
-    const float4* leaf = ctx->lbvh.leaf_pos.as<float4>();
+    const float4* leaf = ctx->bin.pos.as<float4>();
     const int2* child = ctx->lbvh.node_child.as<int2>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
     if (n > 0) {

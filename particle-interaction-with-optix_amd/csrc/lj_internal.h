@@ -51,31 +51,29 @@ struct Status {
 // Workspace of one binned system (or of a concatenated batch).
 struct BinWork {
     DevBuf keys[2];      // u32 radix keys (ping-pong)
-    DevBuf vals[2];      // u32 original index (ping-pong)
     DevBuf hist;         // radix digit histograms
     DevBuf scan_tmp;     // partial sums of the device-wide scans
     DevBuf counts;       // u32 per-cell counts
     DevBuf offsets;      // u32 per-cell offsets (ncells+1)
     DevBuf pos;          // float4 sorted positions, w = original index bits
+    DevBuf pos_alt;      // float4 ping-pong partner of pos during the sort
     DevBuf cell_of;      // u32 cell key per sorted slot
     DevBuf partials;     // per-block partial sums (double) for deterministic totals
     DevBuf counters;     // u64 counters
     DevBuf rows;         // u64 pair-list row offsets
     DevBuf ccounts;      // u32 counts of the cell_edge grid when the force grid is refined
     DevBuf coffsets;     // u32 offsets of the cell_edge grid (ncells+1)
-    int cur = 0;         // which keys/vals buffer holds the sorted result
+    int cur = 0;         // which keys buffer holds the sorted keys
     const uint32_t* sorted_keys() const { return keys[cur].as<uint32_t>(); }
-    const uint32_t* perm() const { return vals[cur].as<uint32_t>(); }
 };
 
+// The LBVH keeps its leaves (positions in Morton order) in BinWork::pos and its
+// Morton codes in BinWork::keys.
 struct LbvhWork {
-    DevBuf morton;        // u32/u64 keys
-    DevBuf order[2];
     DevBuf node_box;      // float4 x2 per node (lo, hi)
     DevBuf node_child;    // int2 per internal node
     DevBuf parent;        // int per node
-    DevBuf flags;         // refit arrival counters
-    DevBuf leaf_pos;      // float4 leaf positions in Morton order
+    DevBuf flags;         // refit arrival counters, then the overflow flag
 };
 
 struct BatchWork {
@@ -147,18 +145,24 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
                      const GridGeom& g, const PairConst& c, float* fx, float* fy, float* fz, float* e,
                      lj_result* result, hipStream_t s);
 
-// Stable LSD radix sort (8-bit digits) of ctx->bin.keys[0] with payload ctx->bin.vals[0],
-// both of n entries; sorts the low `bits` bits.  The result is in keys[cur]/vals[cur].
-int radix_sort(lj_context* ctx, int64_t n, int bits, hipStream_t s);
+// Key functor of the cell grid (for sort_particles in lj_sort.h).
+struct CellKeyOp {
+    GridGeom g;
+    __device__ uint32_t operator()(float x, float y, float z, uint32_t) const { return cell_key(g, x, y, z); }
+};
+
+// Cell counts and offsets (ncells+1) of ctx->bin from the sorted keys of a sort_particles.
+int cell_offsets_from_sorted(lj_context* ctx, int64_t n, uint32_t ncells, hipStream_t s);
+
+// perm[k] = original index of sorted slot k (from ctx->bin.pos after a sort).
+int extract_perm(lj_context* ctx, int64_t n, uint32_t* perm, hipStream_t s);
 
 // res->e_total / res->virial from per-block FP64 partials (pairs of doubles), fixed order.
-void finalize_energy(const double* partials, unsigned nblocks, lj_result* res, hipStream_t s);
-
-// ctx->bin.pos[k] = (x, y, z, bits of i) for i = perm[k] after radix_sort.
-int gather_sorted(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, hipStream_t s);
+// partials must hold room for nblocks + nblocks/1024 + 1 pairs (stage-1 results).
+void finalize_energy(double* partials, unsigned nblocks, lj_result* res, hipStream_t s);
 
 // Bins n particles: fills ctx->bin.offsets (ncells+1), ctx->bin.pos (sorted float4 with
-// the original index in w) and ctx->bin.cell_of.
+// the original index in w) and the sorted cell keys (ctx->bin.sorted_keys()).
 int bin_particles(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
                   const GridGeom& g, hipStream_t s);
 
