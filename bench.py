@@ -167,6 +167,8 @@ def main():
     ap.add_argument("--lbvh", action="store_true", help="time the LBVH path instead of the cell grid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path.",
@@ -236,6 +238,15 @@ def main():
         },
     }
     eng.close()
+    if args.calibration_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line["calibration_copy_bytes"] = 2 * src.numel() * 4
+        del src, dst
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(ljamd, args.workload)

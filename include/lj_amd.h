@@ -88,9 +88,13 @@ typedef struct lj_params {
     int32_t predicate; /* LJ_PRED_* */
     int32_t precision; /* LJ_PREC_* */
     uint32_t flags;    /* LJ_FLAG_* */
-    int32_t refine;    /* force-grid cells per cell_edge along each axis: 0 = choose, 1, 2 or 4;
+    /* Kernel shape (0 = the library's default; lj_autotune picks measured values).
+     * The shape changes speed only: pair counts are identical for every shape, and
+     * forces and energies agree to the rounding of the final FP32 outputs. */
+    int32_t refine;    /* force-grid cells per cell_edge along each axis: 0, 1, 2 or 4;
                           the candidate count keeps the cell_edge grid of the reference */
-    int32_t reserved;  /* must be 0 */
+    int16_t block;     /* particles per workgroup: 0, 64, 128 or 256 */
+    int16_t depth;     /* LDS queue entries per particle: 0, 8, 16 or 32 */
 } lj_params;
 
 /* Scalar results, written to DEVICE memory by the compute calls. */
@@ -207,6 +211,28 @@ int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t nsys, cons
 #define LJ_NUM_PHASES 3
 int lj_profile_enable(lj_context* ctx, int enable);
 int lj_profile_read(lj_context* ctx, double ms[3], unsigned long long calls[3]);
+
+/* Autotuner.  Reference: none -- the reference fixes its launch shape (one thread
+ * per particle in blocks of DefaultNbThreads = 128, cudaLJ.cu:519-520, 647-650) and
+ * its cell edge (= cutoff, cudaLJ.cu:525-531).  lj_autotune times lj_compute on the
+ * given device positions for every valid (refine, block, depth), `reps` calls each
+ * after one untimed call, and writes the fastest shape into p->refine, p->block and
+ * p->depth (no other field changes).  Outputs of the timed calls go to the context's
+ * workspace.  It synchronises `stream` and records hipEvents, so it cannot be
+ * captured in a graph: tune once per workload, then pass the tuned params. */
+typedef struct lj_tuning {
+    int32_t refine, block, depth; /* chosen shape (also written to the params) */
+    int32_t n_tried;              /* shapes timed */
+    float ms_best;                /* mean time of one call with the chosen shape */
+    float ms_default;             /* mean time of one call with the default shape */
+} lj_tuning;
+int lj_autotune(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
+                lj_params* p, int32_t reps, lj_tuning* out, void* stream);
+/* The same for a batch (lj_compute_batched): one shape for the whole batch, written
+ * into every sys[s].params. */
+int lj_autotune_batched(lj_context* ctx, lj_system* sys, int32_t nsys, const float* x,
+                        const float* y, const float* z, int32_t reps, lj_tuning* out,
+                        void* stream);
 
 #ifdef __cplusplus
 }

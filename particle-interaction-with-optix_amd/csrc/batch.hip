@@ -11,18 +11,19 @@
 #include <cstring>
 #include <vector>
 
-#include "lj_internal.h"
-#include "lj_pair.h"
 #include "lj_sort.h"
+#include "lj_stencil.h"
 
 namespace lj {
 
 namespace {
 
 struct SysDesc {
-    GridGeom g;
+    GridGeom g;   // force grid (edge cell_edge / refine); cells [cell_base, +g.ncells) of the key space
+    GridGeom cg;  // cell_edge grid of the candidate count; cells [ccell_base, +cg.ncells)
     PairConst c;
-    uint32_t first, n, cell_base, pad;
+    Trim tr;
+    uint32_t first, n, cell_base, ccell_base;
 };
 
 // Largest s with base[s] <= v (base ascending, base[0] = 0).
@@ -48,21 +49,33 @@ struct BatchKeyOp {
     }
 };
 
-__global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __restrict__ off, uint32_t ncells,
+// Particle counts of the cell_edge grids from the force grids' offsets (refine > 1).
+__global__ __launch_bounds__(256) void k_batch_coarse_counts(const uint32_t* __restrict__ foff, uint32_t nccells,
+                                                             const SysDesc* __restrict__ sys,
+                                                             const uint32_t* __restrict__ ccbase, uint32_t nsys, int K,
+                                                             uint32_t* __restrict__ ccount) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nccells) return;
+    const SysDesc& d = sys[find_sys(ccbase, nsys, c)];
+    ccount[c] = coarse_count(foff + d.cell_base, d.g, d.cg, K, c - d.ccell_base);
+}
+
+// 27-cell candidates per system on the cell_edge grids (offsets coff over the
+// concatenated cell_edge cells), ComputeNbInteractions semantics (cudaLJ.cu:255-297).
+__global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __restrict__ coff, uint32_t nccells,
                                                           const SysDesc* __restrict__ sys,
-                                                          const uint32_t* __restrict__ cbase, uint32_t nsys,
+                                                          const uint32_t* __restrict__ ccbase, uint32_t nsys,
                                                           lj_result* __restrict__ res) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const uint32_t nh = off[c + 1] - off[c];
+    if (c >= nccells) return;
+    const uint32_t nh = coff[c + 1] - coff[c];
     if (!nh) return;
-    const uint32_t s = find_sys(cbase, nsys, c);
-    const GridGeom& g = sys[s].g;
-    const uint32_t base = sys[s].cell_base, local = c - base;
+    const uint32_t s = find_sys(ccbase, nsys, c);
+    const GridGeom& g = sys[s].cg;
+    const uint32_t base = sys[s].ccell_base;
+    int32_t cx, cy, cz;
+    cell_coords(g, c - base, cx, cy, cz);
     const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
-    const int32_t cx = (int32_t)(local % (uint32_t)gx);
-    const int32_t cy = (int32_t)((local / (uint32_t)gx) % (uint32_t)gy);
-    const int32_t cz = (int32_t)(local / ((uint32_t)gx * (uint32_t)gy));
     const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
     unsigned long long sum = 0;
     for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
@@ -70,54 +83,41 @@ __global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __rest
         for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
             if (yy < 0 || yy >= gy) continue;
             const uint32_t row = base + (uint32_t)((zz * gy + yy) * gx);
-            sum += off[row + x1 + 1] - off[row + x0];
+            sum += coff[row + x1 + 1] - coff[row + x0];
         }
     }
     atomicAdd(&res[s].n_candidates, (unsigned long long)nh * (sum - 1));
 }
 
-// One thread per sorted slot, as the single-system kernel, with the grid and pair
-// constants of the slot's system.  Per-particle energy, virial and pair count go to
-// arrays in the caller's order for the per-system reduction.
+// One thread per sorted slot, as the single-system kernel (the same stencil loop and
+// LDS queues), with the grid, trimming and pair constants of the slot's system.  A
+// wave may span several small systems: each lane reads its own system's constants.
+// Per-particle energy, virial and pair count go to arrays in the caller's order for
+// the per-system reduction.
 template <int PRED, int PREC>
-__global__ __launch_bounds__(256) void k_batch_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
-                                                     const uint32_t* __restrict__ off, uint32_t n,
-                                                     const SysDesc* __restrict__ sys, const uint32_t* __restrict__ cbase,
-                                                     uint32_t nsys, float* __restrict__ fx, float* __restrict__ fy,
-                                                     float* __restrict__ fz, float* __restrict__ e,
-                                                     double2* __restrict__ ev, uint32_t* __restrict__ npair) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t key = cell_of[k];
-    const uint32_t s = find_sys(cbase, nsys, key);
-    const GridGeom& g = sys[s].g;
-    const PairConst c = sys[s].c;
-    const uint32_t base = sys[s].cell_base, local = key - base;
+__global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restrict__ pos,
+                                                           const uint32_t* __restrict__ cell_of,
+                                                           const uint32_t* __restrict__ off, uint32_t n,
+                                                           const SysDesc* __restrict__ sys,
+                                                           const uint32_t* __restrict__ cbase, uint32_t nsys, int R,
+                                                           uint32_t depth, float* __restrict__ fx, float* __restrict__ fy,
+                                                           float* __restrict__ fz, float* __restrict__ e,
+                                                           double2* __restrict__ ev, uint32_t* __restrict__ npair) {
+    extern __shared__ uint32_t queue[];
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const bool active = k < n;
+    const uint32_t key = active ? cell_of[k] : 0u;
+    const SysDesc& d = sys[find_sys(cbase, nsys, key)];
+    const GridGeom g = d.g;
+    const PairConst c = d.c;
+    const Trim tr = d.tr;
+    const uint32_t base = d.cell_base;
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, true> acc;
-    const float4 pi = pos[k];
+    const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
-    const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
-    const int32_t cx = (int32_t)(local % (uint32_t)gx);
-    const int32_t cy = (int32_t)((local / (uint32_t)gx) % (uint32_t)gy);
-    const int32_t cz = (int32_t)(local / ((uint32_t)gx * (uint32_t)gy));
-    const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
-    uint32_t np = 0;
-    for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
-        if (zz < 0 || zz >= gz) continue;
-        for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
-            if (yy < 0 || yy >= gy) continue;
-            const uint32_t row = base + (uint32_t)((zz * gy + yy) * gx);
-            const uint32_t b = off[row + x0], en = off[row + x1 + 1];
-            for (uint32_t t = b; t < en; ++t) {
-                const float4 pj = pos[t];
-                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                const float d2 = dist2(dx, dy, dz);
-                if (t == k || !accept<PRED>(c, d2)) continue;
-                ++np;
-                acc.add(c, pj, dx, dy, dz, d2);
-            }
-        }
-    }
+    const uint32_t np = stencil_pairs<PRED, PREC, true>(pos, off, k, active, pi, g, base, key - base, R, tr, c,
+                                                        queue + threadIdx.x, blockDim.x, depth, acc);
+    if (!active) return;
     const uint32_t i = __float_as_uint(pi.w);
     fx[i] = (float)(-c.e24 * (double)acc.ax.value());
     fy[i] = (float)(-c.e24 * (double)acc.ay.value());
@@ -163,30 +163,46 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     if (nsys == 0) return LJ_OK;
     // Plan: per-system geometry, contiguous particle ranges, concatenated cells.
     std::vector<SysDesc> desc((size_t)nsys);
-    std::vector<uint32_t> first((size_t)nsys), cbase((size_t)nsys);
-    uint64_t ntot = 0, ctot = 0;
+    std::vector<uint32_t> first((size_t)nsys), cbase((size_t)nsys), ccbase((size_t)nsys);
+    uint64_t ntot = 0, ctot = 0, cctot = 0;
+    const lj_params& p0 = sys[0].params;
     for (int32_t s = 0; s < nsys; ++s) {
+        const lj_params& ps = sys[s].params;
         GridGeom g;
         PairConst c;
-        LJ_TRY(make_geom(ctx, &sys[s].params, sys[s].n, &g, &c));
-        if (sys[s].params.predicate != sys[0].params.predicate || sys[s].params.precision != sys[0].params.precision)
+        LJ_TRY(make_geom(ctx, &ps, sys[s].n, &g, &c));
+        if (ps.predicate != p0.predicate || ps.precision != p0.precision)
             return fail(ctx, LJ_ERR_INVALID_ARG, "all systems of a batch must share predicate and precision");
+        if (ps.refine != p0.refine || ps.block != p0.block || ps.depth != p0.depth)
+            return fail(ctx, LJ_ERR_INVALID_ARG, "all systems of a batch must share refine, block and depth");
         if (sys[s].first != (int64_t)ntot)
             return fail(ctx, LJ_ERR_INVALID_ARG, "system %d starts at %lld; systems must be packed back to back from 0",
                         s, (long long)sys[s].first);
         SysDesc& d = desc[(size_t)s];
         std::memset(&d, 0, sizeof d);
         d.g = g;
+        d.cg = g;
         d.c = c;
         d.first = (uint32_t)ntot;
         d.n = (uint32_t)sys[s].n;
+        d.ccell_base = (uint32_t)cctot;
+        ntot += (uint64_t)sys[s].n;
+        cctot += g.ncells;
+        if (ntot > 0x7fffffffull) return fail(ctx, LJ_ERR_OVERFLOW, "batch holds more than 2^31-1 particles");
+        if (cctot > 0x7fffffffull) return fail(ctx, LJ_ERR_OVERFLOW, "batch holds more than 2^31-1 cells");
+    }
+    // One shape for the batch: the default refinement from the mean occupancy.
+    const Shape sh = choose_shape(&p0, (int64_t)ntot, cctot);
+    for (int32_t s = 0; s < nsys; ++s) {
+        SysDesc& d = desc[(size_t)s];
+        if (sh.refine > 1) LJ_TRY(refine_geom(ctx, d.cg, &sys[s].params, sh.refine, &d.g));
+        d.tr = make_trim(&sys[s].params, d.g);
         d.cell_base = (uint32_t)ctot;
+        ctot += d.g.ncells;
+        if (ctot > 0x7fffffffull) return fail(ctx, LJ_ERR_OVERFLOW, "batch holds more than 2^31-1 force-grid cells");
         first[(size_t)s] = d.first;
         cbase[(size_t)s] = d.cell_base;
-        ntot += (uint64_t)sys[s].n;
-        ctot += g.ncells;
-        if (ntot > 0x7fffffffull) return fail(ctx, LJ_ERR_OVERFLOW, "batch holds more than 2^31-1 particles");
-        if (ctot > 0x7fffffffull) return fail(ctx, LJ_ERR_OVERFLOW, "batch holds more than 2^31-1 cells");
+        ccbase[(size_t)s] = d.ccell_base;
     }
     if (ntot > 0 && (!x || !y || !z || !fx || !fy || !fz))
         return fail(ctx, LJ_ERR_INVALID_ARG, "positions and forces must not be NULL");
@@ -198,10 +214,11 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     // Upload the plan only when it changed (a cached plan keeps replays free of
     // host-to-device copies).
     const size_t dbytes = desc.size() * sizeof(SysDesc), ubytes = (size_t)nsys * 4;
-    std::vector<uint8_t> blob(dbytes + 2 * ubytes);
+    std::vector<uint8_t> blob(dbytes + 3 * ubytes);
     std::memcpy(blob.data(), desc.data(), dbytes);
     std::memcpy(blob.data() + dbytes, first.data(), ubytes);
     std::memcpy(blob.data() + dbytes + ubytes, cbase.data(), ubytes);
+    std::memcpy(blob.data() + dbytes + 2 * ubytes, ccbase.data(), ubytes);
     if (blob != W.plan) {
         LJ_HIP(ctx, W.sys.grow(blob.size()));
         LJ_HIP(ctx, hipMemcpyAsync(W.sys.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
@@ -209,15 +226,16 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         W.plan = std::move(blob);
     }
     const SysDesc* dsys = W.sys.as<SysDesc>();
-    const uint32_t* dfirst = reinterpret_cast<const uint32_t*>(W.sys.as<uint8_t>() + dbytes);
-    const uint32_t* dcbase = reinterpret_cast<const uint32_t*>(W.sys.as<uint8_t>() + dbytes + ubytes);
+    const uint8_t* dplan = W.sys.as<uint8_t>();
+    const uint32_t* dfirst = reinterpret_cast<const uint32_t*>(dplan + dbytes);
+    const uint32_t* dcbase = reinterpret_cast<const uint32_t*>(dplan + dbytes + ubytes);
+    const uint32_t* dccbase = reinterpret_cast<const uint32_t*>(dplan + dbytes + 2 * ubytes);
 
     const int64_t n = (int64_t)ntot;
-    const uint32_t ncells = (uint32_t)ctot;
+    const uint32_t ncells = (uint32_t)ctot, nccells = (uint32_t)cctot;
     const size_t nn = (size_t)(n > 0 ? n : 1);
     LJ_HIP(ctx, W.partials.grow(nn * sizeof(double2)));
     LJ_HIP(ctx, W.part_sys.grow(nn * sizeof(uint32_t)));
-    const unsigned nb = div_up(n > 0 ? n : 1, 256);
     uint32_t* off;
     {
         PhaseTimer tb(ctx, LJ_PHASE_BIN, s);
@@ -228,34 +246,50 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         LJ_TRY(cell_offsets_from_sorted(ctx, n, ncells, s));
         off = B.offsets.as<uint32_t>();
     }
-    hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(ncells, 256)), dim3(256), 0, s, (const uint32_t*)off, ncells,
-                       dsys, dcbase, (uint32_t)nsys, results);
+    {
+        PhaseTimer to(ctx, LJ_PHASE_OTHER, s);
+        const uint32_t* coff = off;
+        if (sh.refine > 1) {
+            LJ_HIP(ctx, B.ccounts.grow((size_t)nccells * 4));
+            LJ_HIP(ctx, B.coffsets.grow(((size_t)nccells + 1) * 4));
+            hipLaunchKernelGGL(k_batch_coarse_counts, dim3(div_up(nccells, 256)), dim3(256), 0, s, (const uint32_t*)off,
+                               nccells, dsys, dccbase, (uint32_t)nsys, sh.refine, B.ccounts.as<uint32_t>());
+            uint32_t* co = B.coffsets.as<uint32_t>();
+            LJ_TRY(scan_u32(ctx, B.ccounts.as<uint32_t>(), co, nccells, co + nccells, s));
+            coff = co;
+        }
+        hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(nccells, 256)), dim3(256), 0, s, coff, nccells, dsys,
+                           dccbase, (uint32_t)nsys, results);
+    }
     double2* ev = W.partials.as<double2>();
     uint32_t* npair = W.part_sys.as<uint32_t>();
     const float4* pos = B.pos.as<float4>();
     const uint32_t* cell_of = B.sorted_keys();
     {
-    PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
-    if (n > 0) {
-#define LJ_BF(PRED, PREC)                                                                                       \
-    hipLaunchKernelGGL((k_batch_force<PRED, PREC>), dim3(nb), dim3(256), 0, s, pos, cell_of, (const uint32_t*)off, \
-                       (uint32_t)n, dsys, dcbase, (uint32_t)nsys, fx, fy, fz, e, ev, npair)
-#define LJ_BF_PREC(PRED)                                                               \
-    switch (sys[0].params.precision) {                                                 \
-    case LJ_PREC_MIXED: LJ_BF(PRED, LJ_PREC_MIXED); break;                             \
-    case LJ_PREC_FP32: LJ_BF(PRED, LJ_PREC_FP32); break;                               \
-    default: LJ_BF(PRED, LJ_PREC_FP32_COMP); break;                                    \
+        PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
+        const unsigned nb = div_up(n, sh.block);
+        const size_t lds = (size_t)sh.depth * (size_t)sh.block * sizeof(uint32_t);
+        if (n > 0) {
+#define LJ_BF(PRED, PREC)                                                                                         \
+    hipLaunchKernelGGL((k_batch_force<PRED, PREC>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of,               \
+                       (const uint32_t*)off, (uint32_t)n, dsys, dcbase, (uint32_t)nsys, sh.refine, (uint32_t)sh.depth, \
+                       fx, fy, fz, e, ev, npair)
+#define LJ_BF_PREC(PRED)                                   \
+    switch (p0.precision) {                                \
+    case LJ_PREC_MIXED: LJ_BF(PRED, LJ_PREC_MIXED); break; \
+    case LJ_PREC_FP32: LJ_BF(PRED, LJ_PREC_FP32); break;   \
+    default: LJ_BF(PRED, LJ_PREC_FP32_COMP); break;        \
     }
-        switch (sys[0].params.predicate) {
-        case LJ_PRED_AABB: LJ_BF_PREC(LJ_PRED_AABB); break;
-        case LJ_PRED_CUSTOM: LJ_BF_PREC(LJ_PRED_CUSTOM); break;
-        case LJ_PRED_CUDALJ: LJ_BF_PREC(LJ_PRED_CUDALJ); break;
-        case LJ_PRED_SPHERE: LJ_BF_PREC(LJ_PRED_SPHERE); break;
-        case LJ_PRED_TRIANGLE: LJ_BF_PREC(LJ_PRED_TRIANGLE); break;
-        }
+            switch (p0.predicate) {
+            case LJ_PRED_AABB: LJ_BF_PREC(LJ_PRED_AABB); break;
+            case LJ_PRED_CUSTOM: LJ_BF_PREC(LJ_PRED_CUSTOM); break;
+            case LJ_PRED_CUDALJ: LJ_BF_PREC(LJ_PRED_CUDALJ); break;
+            case LJ_PRED_SPHERE: LJ_BF_PREC(LJ_PRED_SPHERE); break;
+            case LJ_PRED_TRIANGLE: LJ_BF_PREC(LJ_PRED_TRIANGLE); break;
+            }
 #undef LJ_BF_PREC
 #undef LJ_BF
-    }
+        }
     }
     {
         PhaseTimer to(ctx, LJ_PHASE_OTHER, s);

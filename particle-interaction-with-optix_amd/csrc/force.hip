@@ -12,96 +12,34 @@
 // of the stencil are one contiguous slot range: the 27 cells are 9 ranges.
 #include <cmath>
 
-#include "lj_internal.h"
-#include "lj_pair.h"
+#include "lj_stencil.h"
 
 namespace lj {
 
 namespace {
 
-__device__ inline void cell_coords(const GridGeom& g, uint32_t key, int32_t& cx, int32_t& cy, int32_t& cz) {
-    cx = (int32_t)(key % (uint32_t)g.g[0]);
-    const uint32_t r = key / (uint32_t)g.g[0];
-    cy = (int32_t)(r % (uint32_t)g.g[1]);
-    cz = (int32_t)(r / (uint32_t)g.g[1]);
-}
-
-// Stencil trimming constants: a row of the stencil is skipped when its (y,z) distance
-// to particle i exceeds the cutoff, and the x-cells of a row are limited to
-// [x_i - w, x_i + w] with w = sqrt(rc^2 - gap^2).  Every bound is widened by `margin`,
-// which exceeds the FP32 rounding of a cell assignment and of x_i +- w, so a row or
-// cell is dropped only when no particle in it can pass the predicate.
-struct Trim {
-    float r2;       // (rc (1 + 2^-16))^2, rounded up
-    float margin;   // absolute widening of every bound
-    float hi[3];    // box_hi: pruning applies to particles inside the box
-    int on;         // 0 for the CUDALJ mode (no cutoff)
-};
-
-// Lower bound of the distance along one axis from coordinate v to cell cc.
-__device__ inline float axis_gap(float v, int32_t cc, int32_t home, float lo, float cw, float margin) {
-    float gap = 0.f;
-    if (cc > home) gap = (lo + (float)cc * cw) - v;
-    else if (cc < home) gap = v - (lo + (float)(cc + 1) * cw);
-    return fmaxf(gap - margin, 0.f);
-}
-
-// One thread per sorted slot k.  The stencil is (2R+1)^3 cells of the force grid
-// (R = 1: the 27 cells of cudaLJ.cu:319-321 on the cell_edge grid; R = 2 or 4: the
-// half- or quarter-edge grid), visited as (2R+1)^2 x-rows in z -> y order, each row
-// a contiguous slot range; rows and x-cells beyond the cutoff are trimmed (Trim).
-// Each row is read two candidates per step, so two position loads are in flight.
+// One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues in
+// dynamic LDS (depth x block words).  Outputs go to the caller's order (pi.w holds the
+// original index); per-block FP64 energy and virial partials are reduced later in a
+// fixed order.
 template <int PRED, int PREC, bool VIR>
-__global__ __launch_bounds__(256) void k_force_simple(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
-                                                      const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int R,
-                                                      Trim tr, PairConst c, float* __restrict__ fx, float* __restrict__ fy,
-                                                      float* __restrict__ fz, float* __restrict__ e,
-                                                      double* __restrict__ partials,
-                                                      unsigned long long* __restrict__ npairs) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
+                                                     const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int R,
+                                                     Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
+                                                     float* __restrict__ fy, float* __restrict__ fz,
+                                                     float* __restrict__ e, double* __restrict__ partials,
+                                                     unsigned long long* __restrict__ npairs) {
+    extern __shared__ uint32_t queue[];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t k = blk * blockDim.x + threadIdx.x;
+    const bool active = k < n;
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR> acc;
-    unsigned long long np = 0;
+    const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    acc.set_i(pi);
+    const uint32_t np = stencil_pairs<PRED, PREC, VIR>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u, R, tr,
+                                                       c, queue + threadIdx.x, blockDim.x, depth, acc);
     double esum = 0.0, vsum = 0.0;
-    if (k < n) {
-        const float4 pi = pos[k];
-        acc.set_i(pi);
-        int32_t cx, cy, cz;
-        cell_coords(g, cell_of[k], cx, cy, cz);
-        const bool trim = tr.on && pi.x >= g.lo[0] && pi.y >= g.lo[1] && pi.z >= g.lo[2] && pi.x <= tr.hi[0] &&
-                          pi.y <= tr.hi[1] && pi.z <= tr.hi[2];
-        const int32_t x0 = cx - R > 0 ? cx - R : 0, x1 = cx + R < g.g[0] - 1 ? cx + R : g.g[0] - 1;
-        const int32_t z0 = cz - R > 0 ? cz - R : 0, z1 = cz + R < g.g[2] - 1 ? cz + R : g.g[2] - 1;
-        const int32_t y0 = cy - R > 0 ? cy - R : 0, y1 = cy + R < g.g[1] - 1 ? cy + R : g.g[1] - 1;
-        auto visit = [&](uint32_t t, float4 pj) {
-            const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            const float d2 = dist2(dx, dy, dz);
-            if (t == k || !accept<PRED>(c, d2)) return;
-            ++np;
-            acc.add(c, pj, dx, dy, dz, d2);
-        };
-        for (int32_t zz = z0; zz <= z1; ++zz) {
-            const float gz = trim ? axis_gap(pi.z, zz, cz, g.lo[2], g.cw, tr.margin) : 0.f;
-            for (int32_t yy = y0; yy <= y1; ++yy) {
-                int32_t xa = x0, xb = x1;
-                if (trim) {
-                    const float gy = axis_gap(pi.y, yy, cy, g.lo[1], g.cw, tr.margin);
-                    const float w2 = tr.r2 - gz * gz - gy * gy;
-                    if (w2 < 0.f) continue;
-                    const float w = __builtin_sqrtf(w2) + tr.margin;
-                    xa = max(x0, axis_cell(pi.x - w, g.lo[0], g.cw, g.gmax[0]));
-                    xb = min(x1, axis_cell(pi.x + w, g.lo[0], g.cw, g.gmax[0]));
-                }
-                const uint32_t row = (uint32_t)((zz * g.g[1] + yy) * g.g[0]);
-                const uint32_t b = off[row + xa], en = off[row + xb + 1];
-                uint32_t t = b;
-                for (; t + 1 < en; t += 2) {
-                    const float4 p0 = pos[t], p1 = pos[t + 1];
-                    visit(t, p0);
-                    visit(t + 1, p1);
-                }
-                if (t < en) visit(t, pos[t]);
-            }
-        }
+    if (active) {
         const uint32_t i = __float_as_uint(pi.w);
         fx[i] = (float)(-c.e24 * (double)acc.ax.value());
         fy[i] = (float)(-c.e24 * (double)acc.ay.value());
@@ -114,33 +52,17 @@ __global__ __launch_bounds__(256) void k_force_simple(const float4* __restrict__
     block_sum2(esum, vsum);
     const unsigned long long bp = block_sum_u64(np);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = esum;
-        partials[2 * blockIdx.x + 1] = vsum;
+        partials[2 * blk] = esum;
+        partials[2 * blk + 1] = vsum;
         if (npairs && bp) atomicAdd(npairs, bp);
     }
 }
 
-// Counts of the cell_edge grid from the refined grid's offsets.  For a refinement by
-// K = 2 or 4, x / (cw/K) is exactly K (x / cw) in binary floating point and
-// floor(floor(K q) / K) = floor(q), so coarse cell X holds fine cells KX .. KX+K-1 (and
-// the extra last fine cells, which the coarse clamp also maps to the last X).
 __global__ __launch_bounds__(256) void k_coarse_counts(const uint32_t* __restrict__ foff, GridGeom fg, GridGeom cg, int K,
                                                        uint32_t* __restrict__ ccount) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cg.ncells) return;
-    const int32_t X = (int32_t)(c % (uint32_t)cg.g[0]);
-    const uint32_t rem = c / (uint32_t)cg.g[0];
-    const int32_t Y = (int32_t)(rem % (uint32_t)cg.g[1]), Z = (int32_t)(rem / (uint32_t)cg.g[1]);
-    const int32_t fx0 = K * X, fx1 = X == cg.g[0] - 1 ? fg.g[0] - 1 : K * X + K - 1;
-    const int32_t fy0 = K * Y, fy1 = Y == cg.g[1] - 1 ? fg.g[1] - 1 : K * Y + K - 1;
-    const int32_t fz0 = K * Z, fz1 = Z == cg.g[2] - 1 ? fg.g[2] - 1 : K * Z + K - 1;
-    uint32_t s = 0;
-    for (int32_t zz = fz0; zz <= fz1; ++zz)
-        for (int32_t yy = fy0; yy <= fy1; ++yy) {
-            const uint32_t row = (uint32_t)((zz * fg.g[1] + yy) * fg.g[0]);
-            s += foff[row + fx1 + 1] - foff[row + fx0];
-        }
-    ccount[c] = s;
+    ccount[c] = coarse_count(foff, fg, cg, K, c);
 }
 
 // FP64 reduction of the per-block partials in a fixed order: stage 1 sums chunks of
@@ -245,12 +167,14 @@ __global__ __launch_bounds__(256) void k_unpack(const float4* __restrict__ pos, 
 }
 
 template <int PRED>
-void launch_force(int prec, bool vir, dim3 grid, hipStream_t s, const float4* pos, const uint32_t* cell_of,
-                  const uint32_t* off, uint32_t n, const GridGeom& g, int R, const Trim& tr, const PairConst& c, float* fx,
-                  float* fy, float* fz, float* e, double* partials, unsigned long long* npairs) {
-#define LJ_LAUNCH(PREC, VIR)                                                                                         \
-    hipLaunchKernelGGL((k_force_simple<PRED, PREC, VIR>), grid, dim3(256), 0, s, pos, cell_of, off, n, g, R, tr, c, fx, \
-                       fy, fz, e, partials, npairs)
+void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t s, const float4* pos,
+                  const uint32_t* cell_of, const uint32_t* off, uint32_t n, const GridGeom& g, const Trim& tr,
+                  const PairConst& c, float* fx, float* fy, float* fz, float* e, double* partials,
+                  unsigned long long* npairs) {
+    const size_t lds = (size_t)sh.depth * (size_t)sh.block * sizeof(uint32_t);
+#define LJ_LAUNCH(PREC, VIR)                                                                                  \
+    hipLaunchKernelGGL((k_force<PRED, PREC, VIR>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
+                       sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs)
     switch (prec) {
     case LJ_PREC_MIXED: if (vir) LJ_LAUNCH(LJ_PREC_MIXED, true); else LJ_LAUNCH(LJ_PREC_MIXED, false); break;
     case LJ_PREC_FP32: if (vir) LJ_LAUNCH(LJ_PREC_FP32, true); else LJ_LAUNCH(LJ_PREC_FP32, false); break;
@@ -288,7 +212,7 @@ static int result_slot(lj_context* ctx, lj_result* user, lj_result** out) {
 }
 
 // Force grid of edge cw/K (K = 2 or 4, exact), G' = (int)(L / (cw/K)).
-static int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int K, GridGeom* f) {
+int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int K, GridGeom* f) {
     *f = g;
     f->cw = g.cw / (float)K;
     uint64_t nc = 1;
@@ -305,17 +229,24 @@ static int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, i
     return LJ_OK;
 }
 
-// Force-grid refinement for a call: an explicit params->refine (1, 2 or 4) wins;
-// otherwise the half-edge grid when the cell_edge grid averages >= 2 particles per
-// cell and the refined grid stays within 2^28 cells.
-static int choose_refine(const lj_params* p, int64_t n, const GridGeom& g) {
-    if (p->predicate == LJ_PRED_CUDALJ) return 1;  // the reference stencil is the 27 cells
-    if (p->refine == 1 || p->refine == 2 || p->refine == 4) return p->refine;
-    const double ppc = (double)n / (double)g.ncells;
-    return (ppc >= 2.0 && (double)g.ncells * 8.0 < (double)(1u << 28)) ? 2 : 1;
+// Shape of a call.  Refinement: an explicit p->refine wins; otherwise the half-edge
+// grid when the cell_edge grid averages >= 2 particles per cell and the refined grid
+// stays within 2^28 cells (the CUDALJ mode keeps the reference's 27 cells).  Block 256
+// and queue depth 16 unless p says otherwise.
+Shape choose_shape(const lj_params* p, int64_t n, uint64_t ncells) {
+    Shape sh;
+    if (p->predicate == LJ_PRED_CUDALJ) sh.refine = 1;
+    else if (p->refine == 1 || p->refine == 2 || p->refine == 4) sh.refine = p->refine;
+    else {
+        const double ppc = (double)n / (double)(ncells ? ncells : 1);
+        sh.refine = (ppc >= 2.0 && (double)ncells * 8.0 < (double)(1u << 28)) ? 2 : 1;
+    }
+    sh.block = p->block ? p->block : 256;
+    sh.depth = p->depth ? p->depth : 16;
+    return sh;
 }
 
-static Trim make_trim(const lj_params* p, const GridGeom& g) {
+Trim make_trim(const lj_params* p, const GridGeom& g) {
     Trim t;
     const double r = (double)p->rc * (1.0 + 0x1p-16);
     t.r2 = std::nextafter((float)(r * r), INFINITY);
@@ -333,7 +264,8 @@ static Trim make_trim(const lj_params* p, const GridGeom& g) {
 int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const lj_params* p,
                      const GridGeom& g, const PairConst& c, float* fx, float* fy, float* fz, float* e,
                      lj_result* result, hipStream_t s) {
-    const int refine = choose_refine(p, n, g);
+    const Shape sh = choose_shape(p, n, g.ncells);
+    const int refine = sh.refine;
     GridGeom fg = g;
     if (refine > 1) LJ_TRY(refine_geom(ctx, g, p, refine, &fg));
     const Trim tr = make_trim(p, fg);
@@ -360,7 +292,7 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
         }
         fill_u64(&res->n_pairs, 1, 0ull, s);
     }
-    const unsigned nb = div_up(n > 0 ? n : 1, 256);
+    const unsigned nb = div_up(n > 0 ? n : 1, sh.block);
     LJ_HIP(ctx, ctx->bin.partials.grow(((size_t)nb + nb / 1024 + 1) * 2 * sizeof(double)));
     double* partials = ctx->bin.partials.as<double>();
     const int prec = p->precision;
@@ -373,7 +305,7 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
         PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
 #define LJ_PRED_CASE(P)                                                                                         \
     case P:                                                                                                     \
-        launch_force<P>(prec, vir, dim3(nb), s, pos, cell_of, off, nn, fg, refine, tr, c, fx, fy, fz, e, partials, \
+        launch_force<P>(prec, vir, sh, nb, s, pos, cell_of, off, nn, fg, tr, c, fx, fy, fz, e, partials,      \
                         &res->n_pairs);                                                                         \
         break;
         switch (p->predicate) {

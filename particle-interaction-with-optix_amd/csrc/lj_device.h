@@ -73,4 +73,13 @@ __host__ __device__ inline bool accept(const PairConst& c, float d2) {
     return false;
 }
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin (workgroup b runs on
+// XCD b % 8), and each XCD has its own L2.  This bijection gives XCD x a contiguous
+// run of logical blocks, so neighbouring blocks, which read the same neighbour cells,
+// share an L2: the first nb % 8 XCDs take q + 1 blocks, the others q = nb / 8.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t xcd = b % 8u, idx = b / 8u, q = nb / 8u, r = nb % 8u;
+    return xcd < r ? xcd * (q + 1u) + idx : r * (q + 1u) + (xcd - r) * q + idx;
+}
+
 }  // namespace lj

@@ -107,6 +107,7 @@ struct lj_context {
     lj::BinWork bin;
     lj::LbvhWork lbvh;
     lj::BatchWork batch;
+    lj::DevBuf tune_out;  // outputs of the autotuner's timed calls
 };
 
 namespace lj {

@@ -110,7 +110,8 @@ __device__ inline T wave_sum(T v) {
     return v;
 }
 
-// Block of 256: deterministic tree sum of two doubles; thread 0 holds the result.
+// Blocks of up to 256 threads (a whole number of waves): deterministic tree sum of two
+// doubles; thread 0 holds the result.
 __device__ inline void block_sum2(double& a, double& b) {
     __shared__ double sh[2][256 / kWaveSize];
     a = wave_sum(a);
@@ -124,7 +125,7 @@ __device__ inline void block_sum2(double& a, double& b) {
     if (threadIdx.x == 0) {
         a = 0.0;
         b = 0.0;
-        for (int q = 0; q < 256 / kWaveSize; ++q) {
+        for (unsigned q = 0; q < blockDim.x / kWaveSize; ++q) {
             a += sh[0][q];
             b += sh[1][q];
         }
@@ -138,7 +139,7 @@ __device__ inline unsigned long long block_sum_u64(unsigned long long v) {
     __syncthreads();
     unsigned long long t = 0;
     if (threadIdx.x == 0)
-        for (int q = 0; q < 256 / kWaveSize; ++q) t += sh[q];
+        for (unsigned q = 0; q < blockDim.x / kWaveSize; ++q) t += sh[q];
     return t;
 }
 

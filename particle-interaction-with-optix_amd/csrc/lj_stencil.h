@@ -1,0 +1,164 @@
+// lj_stencil.h -- the cell-stencil pair loop shared by the single-system and the
+// batched force kernels, and the host-side choices that shape it.
+//
+// Reference: ComputeParticleInterationsParPartNoLoop (Cuda-version/cudaLJ.cu:299-341)
+// visits, for one particle, the particles of the clipped 27 neighbour cells in z -> y
+// -> x order and skips itself.  Cell order is z-major (cudaLJ.cu:127-134), so the
+// x-cells of one (y, z) row of the stencil are one contiguous slot range.
+#pragma once
+
+#include "lj_internal.h"
+#include "lj_pair.h"
+
+namespace lj {
+
+// Stencil trimming constants: a row of the stencil is skipped when its (y,z) distance
+// to particle i exceeds the cutoff, and the x-cells of a row are limited to
+// [x_i - w, x_i + w] with w = sqrt(rc^2 - gap^2).  Every bound is widened by `margin`,
+// which exceeds the FP32 rounding of a cell assignment and of x_i +- w, so a row or
+// cell is dropped only when no particle in it can pass the predicate.
+struct Trim {
+    float r2;       // (rc (1 + 2^-16))^2, rounded up
+    float margin;   // absolute widening of every bound
+    float hi[3];    // box_hi: pruning applies to particles inside the box
+    int on;         // 0 for the CUDALJ mode (no cutoff)
+};
+
+// Kernel shape of a force launch: force-grid refinement K (stencil radius K cells of
+// edge cell_edge/K), particles per workgroup and LDS queue entries per particle.
+struct Shape {
+    int refine, block, depth;
+};
+
+constexpr int kMaxBlock = 256;
+constexpr int kMaxDepth = 32;
+
+__device__ inline void cell_coords(const GridGeom& g, uint32_t key, int32_t& cx, int32_t& cy, int32_t& cz) {
+    cx = (int32_t)(key % (uint32_t)g.g[0]);
+    const uint32_t r = key / (uint32_t)g.g[0];
+    cy = (int32_t)(r % (uint32_t)g.g[1]);
+    cz = (int32_t)(r / (uint32_t)g.g[1]);
+}
+
+// Lower bound of the distance along one axis from coordinate v to cell cc.
+__device__ inline float axis_gap(float v, int32_t cc, int32_t home, float lo, float cw, float margin) {
+    float gap = 0.f;
+    if (cc > home) gap = (lo + (float)cc * cw) - v;
+    else if (cc < home) gap = v - (lo + (float)(cc + 1) * cw);
+    return fmaxf(gap - margin, 0.f);
+}
+
+// Particles in cell c of the cell_edge grid cg, from the offsets foff of the refined
+// grid fg (edge cg.cw / K).  For K = 2 or 4, x / (cw/K) is exactly K (x / cw) in binary
+// floating point and floor(floor(K q) / K) = floor(q), so coarse cell X holds fine
+// cells KX .. KX+K-1 (and the extra last fine cells, which the coarse clamp also maps
+// to the last X).
+__device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const GridGeom& fg, const GridGeom& cg, int K,
+                                        uint32_t c) {
+    const int32_t X = (int32_t)(c % (uint32_t)cg.g[0]);
+    const uint32_t rem = c / (uint32_t)cg.g[0];
+    const int32_t Y = (int32_t)(rem % (uint32_t)cg.g[1]), Z = (int32_t)(rem / (uint32_t)cg.g[1]);
+    const int32_t fx0 = K * X, fx1 = X == cg.g[0] - 1 ? fg.g[0] - 1 : K * X + K - 1;
+    const int32_t fy0 = K * Y, fy1 = Y == cg.g[1] - 1 ? fg.g[1] - 1 : K * Y + K - 1;
+    const int32_t fz0 = K * Z, fz1 = Z == cg.g[2] - 1 ? fg.g[2] - 1 : K * Z + K - 1;
+    uint32_t s = 0;
+    for (int32_t zz = fz0; zz <= fz1; ++zz)
+        for (int32_t yy = fy0; yy <= fy1; ++yy) {
+            const uint32_t row = (uint32_t)((zz * fg.g[1] + yy) * fg.g[0]);
+            s += foff[row + fx1 + 1] - foff[row + fx0];
+        }
+    return s;
+}
+
+// Accepted partners of the particle in sorted slot k (position pi, cell `key` of grid
+// g, whose cell 0 is global cell `cbase`), summed into acc; returns their number.
+// The stencil is (2R+1)^3 cells of g, visited as (2R+1)^2 x-rows in z -> y order,
+// rows and x-cells beyond the cutoff trimmed.
+//
+// The candidate loop only evaluates the predicate.  An accepted partner's slot goes
+// to the lane's queue in LDS (entry q of the lane at queue[q * stride]: lanes of a
+// wave store to consecutive words, never a bank conflict), and the pair arithmetic
+// runs when some lane's queue is nearly full, on every lane's queue at once.
+// Evaluated inside the candidate loop, the FP64 pair chain would run for the whole
+// wave whenever any one lane accepts (a quarter of the lanes at the cutoff densities
+// of interest); from the queues it runs with most lanes busy.  Queues are emptied in
+// order, so a particle's sum keeps the stencil order of its partners whatever the
+// queue depth.  Every lane of the wave must call this (inactive lanes with
+// active = false), because the flushes are wave-wide.
+template <int PRED, int PREC, bool VIR>
+__device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
+                                         bool active, const float4& pi, const GridGeom& g, uint32_t cbase,
+                                         uint32_t key, int R, const Trim& tr, const PairConst& c,
+                                         uint32_t* __restrict__ queue, uint32_t stride, unsigned depth,
+                                         PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR>& acc) {
+    unsigned pc = 0;  // queued partners of this lane
+    uint32_t np = 0;
+    const uint32_t kload = active ? k : 0;  // an in-bounds slot for idle loads
+    auto flush = [&]() {
+        for (unsigned q = 0; __any(q < pc); q += 2) {
+            const bool a0 = q < pc, a1 = q + 1 < pc;
+            const float4 p0 = pos[a0 ? queue[q * stride] : kload];
+            const float4 p1 = pos[a1 ? queue[(q + 1) * stride] : kload];
+            if (a0) {
+                const float dx = p0.x - pi.x, dy = p0.y - pi.y, dz = p0.z - pi.z;
+                acc.add(c, p0, dx, dy, dz, dist2(dx, dy, dz));
+            }
+            if (a1) {
+                const float dx = p1.x - pi.x, dy = p1.y - pi.y, dz = p1.z - pi.z;
+                acc.add(c, p1, dx, dy, dz, dist2(dx, dy, dz));
+            }
+        }
+        pc = 0;
+    };
+    auto visit = [&](uint32_t t, const float4& pj) {
+        const float d2 = dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+        if (t == k || !accept<PRED>(c, d2)) return;
+        queue[pc * stride] = t;
+        ++pc;
+        ++np;
+    };
+    int32_t cx = 0, cy = 0, cz = 0;
+    if (active) cell_coords(g, key, cx, cy, cz);
+    const bool trim = active && tr.on && pi.x >= g.lo[0] && pi.y >= g.lo[1] && pi.z >= g.lo[2] &&
+                      pi.x <= tr.hi[0] && pi.y <= tr.hi[1] && pi.z <= tr.hi[2];
+    const int32_t x0 = cx - R > 0 ? cx - R : 0, x1 = cx + R < g.g[0] - 1 ? cx + R : g.g[0] - 1;
+    // an inactive lane has an empty stencil (z0 > z1) but still takes part in the flushes
+    const int32_t z0 = active ? (cz - R > 0 ? cz - R : 0) : 1;
+    const int32_t z1 = active ? (cz + R < g.g[2] - 1 ? cz + R : g.g[2] - 1) : 0;
+    const int32_t y0 = cy - R > 0 ? cy - R : 0, y1 = cy + R < g.g[1] - 1 ? cy + R : g.g[1] - 1;
+    for (int32_t zz = z0; zz <= z1; ++zz) {
+        const float gz = trim ? axis_gap(pi.z, zz, cz, g.lo[2], g.cw, tr.margin) : 0.f;
+        for (int32_t yy = y0; yy <= y1; ++yy) {
+            int32_t xa = x0, xb = x1;
+            if (trim) {
+                const float gy = axis_gap(pi.y, yy, cy, g.lo[1], g.cw, tr.margin);
+                const float w2 = tr.r2 - gz * gz - gy * gy;
+                if (w2 < 0.f) continue;
+                const float w = __builtin_sqrtf(w2) + tr.margin;
+                xa = max(x0, axis_cell(pi.x - w, g.lo[0], g.cw, g.gmax[0]));
+                xb = min(x1, axis_cell(pi.x + w, g.lo[0], g.cw, g.gmax[0]));
+            }
+            const uint32_t row = cbase + (uint32_t)((zz * g.g[1] + yy) * g.g[0]);
+            const uint32_t b = off[row + xa], en = off[row + xb + 1];
+            for (uint32_t t = b; t < en; t += 2) {
+                if (__any(pc + 2 > depth)) flush();
+                const float4 p0 = pos[t];
+                const float4 p1 = pos[t + 1 < en ? t + 1 : t];
+                visit(t, p0);
+                if (t + 1 < en) visit(t + 1, p1);
+            }
+        }
+    }
+    // rows end at different times in different lanes: flush while any queue holds pairs
+    while (__any(pc > 0)) flush();
+    return np;
+}
+
+// Host side (force.hip).
+int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int K, GridGeom* f);
+Trim make_trim(const lj_params* p, const GridGeom& g);
+// Shape of a call over n particles in ncells cells of the cell_edge grid: explicit
+// fields of p (validated by make_geom) win; zero fields take the defaults.
+Shape choose_shape(const lj_params* p, int64_t n, uint64_t ncells);
+
+}  // namespace lj

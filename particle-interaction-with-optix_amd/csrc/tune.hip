@@ -1,0 +1,164 @@
+// tune.hip -- the autotuner: measured choice of the force-kernel shape.
+//
+// The shape of a force launch is the force-grid refinement K (cells of edge
+// cell_edge / K, stencil radius K), the particles per workgroup and the depth of the
+// per-lane LDS queues.  None of them changes which pairs are found; they trade
+// candidate tests (fewer with finer cells) against cell-offset reads and loop
+// overhead (more with finer cells), and occupancy against queue fill.  The best
+// choice depends on the density and distribution of the particles, so it is
+// measured: every valid shape runs `reps` times on the caller's positions after one
+// untimed call, timed with hipEvents on the caller's stream, and the fastest wins.
+#include <vector>
+
+#include "lj_stencil.h"
+
+namespace lj {
+namespace {
+
+constexpr int kRefines[] = {1, 2, 4};
+constexpr int kBlocks[] = {64, 128, 256};
+constexpr int kDepths[] = {8, 16, 32};
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// Mean milliseconds of `reps` calls of fn after one untimed call; a negative value is
+// the status of a failing call.
+template <class Fn>
+int time_calls(lj_context* ctx, EventPair& ev, hipStream_t s, int reps, Fn fn, float* ms) {
+    LJ_TRY(fn());
+    LJ_HIP(ctx, hipEventRecord(ev.a, s));
+    for (int r = 0; r < reps; ++r) LJ_TRY(fn());
+    LJ_HIP(ctx, hipEventRecord(ev.b, s));
+    LJ_HIP(ctx, hipEventSynchronize(ev.b));
+    float t = 0.f;
+    LJ_HIP(ctx, hipEventElapsedTime(&t, ev.a, ev.b));
+    *ms = t / (float)reps;
+    return LJ_OK;
+}
+
+// Restores the profiler switch on every exit path: tuning calls are not profiled.
+struct ProfilerOff {
+    lj_context* ctx;
+    bool was;
+    explicit ProfilerOff(lj_context* c) : ctx(c), was(c->prof.on) { c->prof.on = false; }
+    ~ProfilerOff() { ctx->prof.on = was; }
+};
+
+template <class Run>
+int search(lj_context* ctx, hipStream_t s, int reps, bool cudalj, Run run, lj_tuning* best) {
+    EventPair ev;
+    LJ_HIP(ctx, hipEventCreate(&ev.a));
+    LJ_HIP(ctx, hipEventCreate(&ev.b));
+    ProfilerOff off(ctx);
+    float ms = 0.f;
+    LJ_TRY(time_calls(ctx, ev, s, reps, [&] { return run(0, 0, 0); }, &ms));
+    *best = lj_tuning{};
+    best->ms_default = ms;
+    best->ms_best = -1.f;
+    for (int K : kRefines) {
+        if (cudalj && K != 1) continue;  // the reference stencil is the 27 cells
+        for (int b : kBlocks)
+            for (int d : kDepths) {
+                const int r = time_calls(ctx, ev, s, reps, [&] { return run(K, b, d); }, &ms);
+                if (r == LJ_ERR_OVERFLOW) goto next_refine;  // finer grid too large for this workload
+                LJ_TRY(r);
+                ++best->n_tried;
+                if (best->ms_best < 0.f || ms < best->ms_best) {
+                    best->ms_best = ms;
+                    best->refine = K;
+                    best->block = b;
+                    best->depth = d;
+                }
+            }
+    next_refine:;
+    }
+    if (best->n_tried == 0) return fail(ctx, LJ_ERR_INVALID_ARG, "no valid kernel shape for this workload");
+    return LJ_OK;
+}
+
+}  // namespace
+}  // namespace lj
+
+using namespace lj;
+
+extern "C" {
+
+int lj_autotune(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, lj_params* p, int32_t reps,
+                lj_tuning* out, void* stream) {
+    if (!ctx) return LJ_ERR_INVALID_ARG;
+    GridGeom g;
+    PairConst c;
+    LJ_TRY(make_geom(ctx, p, n, &g, &c));
+    if (reps < 1 || reps > 1000) return fail(ctx, LJ_ERR_INVALID_ARG, "reps must be in [1, 1000] (got %d)", reps);
+    if (n <= 0) return fail(ctx, LJ_ERR_INVALID_ARG, "autotuning needs particles (n = %lld)", (long long)n);
+    if (!x || !y || !z) return fail(ctx, LJ_ERR_INVALID_ARG, "positions must not be NULL");
+    LJ_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nn = (size_t)n;
+    LJ_HIP(ctx, ctx->tune_out.grow(3 * nn * sizeof(float) + sizeof(lj_result) + 256));
+    float* f = ctx->tune_out.as<float>();
+    lj_result* res = reinterpret_cast<lj_result*>(ctx->tune_out.as<uint8_t>() + ((3 * nn * sizeof(float) + 255) & ~size_t(255)));
+    lj_params q = *p;
+    auto run = [&](int K, int b, int d) {
+        q.refine = K;
+        q.block = (int16_t)b;
+        q.depth = (int16_t)d;
+        GridGeom gq;
+        PairConst cq;
+        LJ_TRY(make_geom(ctx, &q, n, &gq, &cq));
+        return compute_cellgrid(ctx, x, y, z, n, &q, gq, cq, f, f + nn, f + 2 * nn, nullptr, res, s);
+    };
+    lj_tuning best;
+    LJ_TRY(search(ctx, s, reps, p->predicate == LJ_PRED_CUDALJ, run, &best));
+    p->refine = best.refine;
+    p->block = (int16_t)best.block;
+    p->depth = (int16_t)best.depth;
+    if (out) *out = best;
+    return LJ_OK;
+}
+
+int lj_autotune_batched(lj_context* ctx, lj_system* sys, int32_t nsys, const float* x, const float* y, const float* z,
+                        int32_t reps, lj_tuning* out, void* stream) {
+    if (!ctx) return LJ_ERR_INVALID_ARG;
+    if (nsys <= 0 || !sys) return fail(ctx, LJ_ERR_INVALID_ARG, "autotuning needs a non-empty system array");
+    if (reps < 1 || reps > 1000) return fail(ctx, LJ_ERR_INVALID_ARG, "reps must be in [1, 1000] (got %d)", reps);
+    int64_t ntot = 0;
+    for (int32_t i = 0; i < nsys; ++i) {
+        if (sys[i].n < 0) return fail(ctx, LJ_ERR_INVALID_ARG, "system %d has n < 0", i);
+        ntot += sys[i].n;
+    }
+    if (ntot <= 0 || ntot > 0x7fffffffll) return fail(ctx, LJ_ERR_INVALID_ARG, "batch particle count out of range");
+    LJ_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nn = (size_t)ntot;
+    const size_t fbytes = (3 * nn * sizeof(float) + 255) & ~size_t(255);
+    LJ_HIP(ctx, ctx->tune_out.grow(fbytes + (size_t)nsys * sizeof(lj_result)));
+    float* f = ctx->tune_out.as<float>();
+    lj_result* res = reinterpret_cast<lj_result*>(ctx->tune_out.as<uint8_t>() + fbytes);
+    std::vector<lj_system> q(sys, sys + nsys);
+    auto run = [&](int K, int b, int d) {
+        for (auto& t : q) {
+            t.params.refine = K;
+            t.params.block = (int16_t)b;
+            t.params.depth = (int16_t)d;
+        }
+        return lj_compute_batched(ctx, q.data(), nsys, x, y, z, f, f + nn, f + 2 * nn, nullptr, res, stream);
+    };
+    lj_tuning best;
+    LJ_TRY(search(ctx, s, reps, sys[0].params.predicate == LJ_PRED_CUDALJ, run, &best));
+    for (int32_t i = 0; i < nsys; ++i) {
+        sys[i].params.refine = best.refine;
+        sys[i].params.block = (int16_t)best.block;
+        sys[i].params.depth = (int16_t)best.depth;
+    }
+    if (out) *out = best;
+    return LJ_OK;
+}
+
+}  // extern "C"

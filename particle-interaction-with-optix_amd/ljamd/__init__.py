@@ -55,6 +55,8 @@ EXPORTS = (
     "lj_compute_batched",
     "lj_profile_enable",
     "lj_profile_read",
+    "lj_autotune",
+    "lj_autotune_batched",
 )
 PHASES = ("bin", "force", "other")
 
@@ -72,7 +74,8 @@ class LJParams(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("flags", ctypes.c_uint32),
         ("refine", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("block", ctypes.c_int16),
+        ("depth", ctypes.c_int16),
     ]
 
 
@@ -87,6 +90,20 @@ class LJResult(ctypes.Structure):
 
 class LJSystem(ctypes.Structure):
     _fields_ = [("first", ctypes.c_int64), ("n", ctypes.c_int64), ("params", LJParams)]
+
+
+class LJTuning(ctypes.Structure):
+    _fields_ = [
+        ("refine", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("depth", ctypes.c_int32),
+        ("n_tried", ctypes.c_int32),
+        ("ms_best", ctypes.c_float),
+        ("ms_default", ctypes.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 RESULT_DTYPE = np.dtype(
@@ -137,6 +154,8 @@ def load(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
         "lj_compute_batched": ([vp, P(LJSystem), ctypes.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "lj_profile_enable": ([vp, ctypes.c_int], ctypes.c_int),
         "lj_profile_read": ([vp, P(ctypes.c_double), P(ctypes.c_ulonglong)], ctypes.c_int),
+        "lj_autotune": ([vp, vp, vp, vp, i64, P(LJParams), ctypes.c_int32, P(LJTuning), vp], ctypes.c_int),
+        "lj_autotune_batched": ([vp, P(LJSystem), ctypes.c_int32, vp, vp, vp, ctypes.c_int32, P(LJTuning), vp], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         if hasattr(lib, name):  # tests check that every declared symbol is exported
@@ -160,8 +179,11 @@ def make_params(
     precision: str | int = "mixed",
     flags: int = 0,
     refine: int = 0,
+    block: int = 0,
+    depth: int = 0,
 ) -> LJParams:
-    """Run parameters; sigma defaults to rc / 2.5 (the workloads' choice)."""
+    """Run parameters; sigma defaults to rc / 2.5 (the workloads' choice).  refine,
+    block and depth are the kernel shape (0 = default; see Engine.autotune)."""
     p = LJParams()
     p.rc = rc
     p.sigma = rc / 2.5 if sigma is None else sigma
@@ -175,7 +197,8 @@ def make_params(
     p.precision = PREC[precision] if isinstance(precision, str) else int(precision)
     p.flags = flags
     p.refine = refine
-    p.reserved = 0
+    p.block = block
+    p.depth = depth
     return p
 
 
@@ -192,6 +215,8 @@ def params_to_dict(p: LJParams) -> dict:
         "precision": p.precision,
         "flags": p.flags,
         "refine": p.refine,
+        "block": p.block,
+        "depth": p.depth,
     }
 
 
@@ -366,6 +391,28 @@ class Engine:
                                         _ptr(out["fz"]), _ptr(out["e"]), _ptr(out["result"]), self._stream())
         )
         return out
+
+    def autotune(self, x, y, z, p: LJParams, reps: int = 3) -> dict:
+        """Times every valid kernel shape on these positions and writes the fastest
+        into p (refine, block, depth); returns the lj_tuning record as a dict."""
+        t = LJTuning()
+        self._check(
+            self.lib.lj_autotune(self.ctx, _ptr(x), _ptr(y), _ptr(z), x.numel(), ctypes.byref(p), reps, ctypes.byref(t), self._stream())
+        )
+        return t.as_dict()
+
+    def autotune_batched(self, systems: Sequence[LJSystem], x, y, z, reps: int = 3) -> dict:
+        """One shape for a whole batch, written into every system's params."""
+        arr = (LJSystem * len(systems))(*systems)
+        t = LJTuning()
+        self._check(
+            self.lib.lj_autotune_batched(self.ctx, arr, len(systems), _ptr(x), _ptr(y), _ptr(z), reps, ctypes.byref(t), self._stream())
+        )
+        for q, sy in enumerate(systems):
+            sy.params.refine = arr[q].params.refine
+            sy.params.block = arr[q].params.block
+            sy.params.depth = arr[q].params.depth
+        return t.as_dict()
 
 
 def version() -> str:

@@ -33,6 +33,9 @@ def test_struct_layouts_match_header_sizes():
     assert ctypes.sizeof(ljamd.LJParams) == 64
     assert ctypes.sizeof(ljamd.LJResult) == 32
     assert ctypes.sizeof(ljamd.LJSystem) == 16 + 64
+    assert ctypes.sizeof(ljamd.LJTuning) == 24
+    assert ljamd.LJParams.refine.offset == 56
+    assert ljamd.LJParams.block.offset == 60 and ljamd.LJParams.depth.offset == 62
     # the oracle's params are the leading fields of the engine's (same offsets)
     for name, _ in po.OrcParams._fields_:
         assert getattr(po.OrcParams, name).offset == getattr(ljamd.LJParams, name).offset
@@ -56,6 +59,9 @@ def test_grid_dims_rejects_bad_params():
         ljamd.make_params(1e-9),  # more than 2^24 cells per axis
         ljamd.make_params(0.1, predicate=17),
         ljamd.make_params(0.1, precision=9),
+        ljamd.make_params(0.1, refine=3),
+        ljamd.make_params(0.1, block=96),
+        ljamd.make_params(0.1, depth=12),
     ):
         with pytest.raises(ljamd.LJError):
             ljamd.grid_dims(p)

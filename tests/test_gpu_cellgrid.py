@@ -120,6 +120,59 @@ def test_refined_grids_same_pairs_and_forces(engine, name, refine):
     assert err.max() <= FORCE_TOL
 
 
+@pytest.mark.parametrize("name", ["c1_uniform_100k", "c5_clusters_64k", "tiny_7"])
+def test_kernel_shapes_same_results(engine, name):
+    """Workgroup size and LDS queue depth change neither the pairs nor the order in
+    which each particle's partners are summed: outputs are bit-identical across them
+    at a fixed refinement."""
+    (x, y, z), p = _case(name)
+    p.refine = 2
+    print(f"[shapes] {name}", flush=True)
+    base = None
+    for block in (64, 128, 256):
+        for depth in (8, 16, 32):
+            p.block, p.depth = block, depth
+            f, e, res = _forces(engine, x, y, z, p)
+            cur = (np.stack(f).view(np.uint32), e.view(np.uint32), int(res["n_pairs"]), int(res["n_candidates"]),
+                   float(res["e_total"]))
+            if base is None:
+                base = cur
+                continue
+            assert np.array_equal(cur[0], base[0]), (block, depth)
+            assert np.array_equal(cur[1], base[1]), (block, depth)
+            assert cur[2:4] == base[2:4], (block, depth)
+            # the FP64 total is summed per workgroup, so it may differ in the last bits
+            assert cur[4] == pytest.approx(base[4], rel=1e-12, abs=1e-300)
+
+
+def test_autotune_picks_a_valid_shape(engine):
+    """lj_autotune times the shapes, writes a valid one into the params, and the tuned
+    params give the same pairs and forces within the bar."""
+    (x, y, z), p = _case("c1_uniform_100k")
+    dx, dy, dz = engine.to_device(x, y, z)
+    print("[autotune] c1", flush=True)
+    t = engine.autotune(dx, dy, dz, p, reps=2)
+    assert t["refine"] in (1, 2, 4) and t["block"] in (64, 128, 256) and t["depth"] in (8, 16, 32)
+    assert (p.refine, p.block, p.depth) == (t["refine"], t["block"], t["depth"])
+    assert t["n_tried"] == 27
+    assert 0 < t["ms_best"] and 0 < t["ms_default"]
+    ref = po.forces_f64(po.from_lj(p), x, y, z)
+    f, e, res = _forces(engine, x, y, z, p)
+    assert int(res["n_pairs"]) == ref["n_pairs"]
+    assert int(res["n_candidates"]) == ref["n_candidates"]
+    err = po.rel_errors(f, (ref["fx"], ref["fy"], ref["fz"]))["rel"]
+    assert err.max() <= FORCE_TOL
+
+
+def test_autotune_cudalj_keeps_the_27_cells(engine):
+    (x, y, z), p = _case("c1_uniform_100k")
+    q = ljamd.make_params(float(p.rc), sigma=1.0, softening2=1e-5, predicate="cudalj")
+    dx, dy, dz = engine.to_device(x[:20000], y[:20000], z[:20000])
+    t = engine.autotune(dx, dy, dz, q, reps=1)
+    assert t["refine"] == 1 and q.refine == 1
+    assert t["n_tried"] == 9
+
+
 def test_trimming_at_cell_faces(engine):
     """Pairs at exactly the cutoff across cell faces and corners, for every refinement."""
     rc = np.float32(0.0625)  # cell faces at exact binary fractions
@@ -217,6 +270,8 @@ def test_invalid_params_rejected(engine):
         ljamd.make_params(0.1, predicate=9),
         ljamd.make_params(1e-9),
         ljamd.make_params(0.1, refine=3),
+        ljamd.make_params(0.1, block=96),
+        ljamd.make_params(0.1, depth=12),
     ):
         with pytest.raises(ljamd.LJError):
             engine.compute(dx, dy, dz, bad)

@@ -88,8 +88,14 @@ def _small_batch(nsys=48):
     return workloads.batch_c4(nsys=nsys, nmin=64, nmax=4096, seed=77)
 
 
-def test_batched_equals_single_calls_and_oracle(engine):
+@pytest.mark.parametrize("refine", [1, 2, 4])
+def test_batched_equals_single_calls_and_oracle(engine, refine):
+    """One launch over all systems gives each system the bits of its own call with
+    the same kernel shape, and the oracle's counts, totals and forces."""
     systems, X, Y, Z = _small_batch()
+    for sy in systems:
+        sy.params.refine = refine
+    print(f"[batched] refine={refine}", flush=True)
     dx, dy, dz = engine.to_device(X, Y, Z)
     out = engine.compute_batched(systems, dx, dy, dz)
     engine.torch.cuda.synchronize()
@@ -109,6 +115,37 @@ def test_batched_equals_single_calls_and_oracle(engine):
         assert int(res[s]["n_candidates"]) == ref["n_candidates"]
         assert res[s]["e_total"] == pytest.approx(ref["e_total"], rel=1e-9)
         assert res[s]["virial"] == pytest.approx(ref["virial"], rel=1e-9)
+        err = po.rel_errors([arr[a:b] for arr in fb[:3]], (ref["fx"], ref["fy"], ref["fz"]))["rel"]
+        assert err.max() <= 1e-5, s
+
+
+def test_batched_autotune(engine):
+    """lj_autotune_batched writes one valid shape into every system; the tuned batch
+    finds the same pairs as the default one."""
+    systems, X, Y, Z = _small_batch(64)
+    dx, dy, dz = engine.to_device(X, Y, Z)
+    out = engine.compute_batched(systems, dx, dy, dz)
+    engine.torch.cuda.synchronize()
+    res0 = ljamd.read_results(out["result"], len(systems)).copy()
+    print("[batched autotune]", flush=True)
+    t = engine.autotune_batched(systems, dx, dy, dz, reps=2)
+    assert t["refine"] in (1, 2, 4) and t["block"] in (64, 128, 256) and t["depth"] in (8, 16, 32)
+    assert all((sy.params.refine, sy.params.block, sy.params.depth) == (t["refine"], t["block"], t["depth"])
+               for sy in systems)
+    out = engine.compute_batched(systems, dx, dy, dz)
+    engine.torch.cuda.synchronize()
+    res1 = ljamd.read_results(out["result"], len(systems))
+    assert np.array_equal(res0["n_pairs"], res1["n_pairs"])
+    assert np.array_equal(res0["n_candidates"], res1["n_candidates"])
+    assert np.allclose(res0["e_total"], res1["e_total"], rtol=1e-9, atol=0)
+
+
+def test_batched_rejects_mixed_shapes(engine):
+    systems, X, Y, Z = _small_batch(4)
+    systems[1].params.depth = 8
+    dx, dy, dz = engine.to_device(X, Y, Z)
+    with pytest.raises(ljamd.LJError):
+        engine.compute_batched(systems, dx, dy, dz)
 
 
 def test_batched_graph_replays_match_eager(engine):
