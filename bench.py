@@ -12,6 +12,9 @@ ordered pairs / step time.  The JSON line also carries:
                 same-density sample (bounded to a few seconds of CPU work)
   batched       the 4,096-system batch: one call over all systems (eager and
                 replayed from a captured graph) against one call per system
+Before the timed steps the kernel shape (force-grid refinement, particles per
+workgroup, LDS queue depth) is picked by the engine's autotuner on the same input,
+outside the timed region; the chosen shape is reported under config.shape.
 """
 from __future__ import annotations
 
@@ -91,7 +94,7 @@ def cpu_baseline(ljamd, workload: str):
             "sample": sample + f", {reps} reps, {t_total:.1f} s"}
 
 
-def batched(torch, ljamd, steps):
+def batched(torch, ljamd, steps, autotune=True):
     from ljamd import workloads
 
     t0 = time.perf_counter()
@@ -99,6 +102,11 @@ def batched(torch, ljamd, steps):
     log(f"c4: {len(systems)} systems, {X.size} particles generated in {time.perf_counter() - t0:.1f} s")
     eng = ljamd.Engine(0)
     dx, dy, dz = eng.to_device(X, Y, Z)
+    tuning = None
+    if autotune:
+        t0 = time.perf_counter()
+        tuning = eng.autotune_batched(systems, dx, dy, dz, reps=2)
+        log(f"c4 autotune {tuning} in {time.perf_counter() - t0:.1f} s")
     out = eng.compute_batched(systems, dx, dy, dz)
     torch.cuda.synchronize()
     t_batch = time_steps(torch, lambda: eng.compute_batched(systems, dx, dy, dz, out=out), steps)
@@ -154,6 +162,8 @@ def batched(torch, ljamd, steps):
         "systems_per_s_batched": nsys / (best / 1e3),
         "systems_per_s_serial": nsys / (t_serial / 1e3),
         "speedup": t_serial / best,
+        "shape": {"refine": systems[0].params.refine, "block": systems[0].params.block, "depth": systems[0].params.depth},
+        "autotune": tuning,
     }
 
 
@@ -167,6 +177,9 @@ def main():
     ap.add_argument("--lbvh", action="store_true", help="time the LBVH path instead of the cell grid")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default kernel shape")
+    ap.add_argument("--shape", default=None, metavar="REFINE,BLOCK,DEPTH",
+                    help="fixed kernel shape for the single-system steps instead of autotuning")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -186,6 +199,13 @@ def main():
     eng = ljamd.Engine(0)
     dx, dy, dz = eng.to_device(w.x, w.y, w.z)
     n = w.x.size
+    tuning = None
+    if args.shape:
+        w.params.refine, w.params.block, w.params.depth = (int(v) for v in args.shape.split(","))
+    elif not args.no_autotune and not args.lbvh:
+        t0 = time.perf_counter()
+        tuning = eng.autotune(dx, dy, dz, w.params, reps=3)
+        log(f"autotune {tuning} in {time.perf_counter() - t0:.1f} s")
     out = eng.compute(dx, dy, dz, w.params, lbvh=args.lbvh)
 
     def step():
@@ -226,6 +246,8 @@ def main():
             "ms_force": ms_force,
             "pairs_per_s_force_kernel": pairs / (ms_force / 1e3),
             "e_total": float(res["e_total"]),
+            "shape": {"refine": w.params.refine, "block": w.params.block, "depth": w.params.depth},
+            "autotune": tuning,
         },
         "roofline": {
             "bound": "mfma",
@@ -252,7 +274,7 @@ def main():
         line["cpu_baseline"] = cpu_baseline(ljamd, args.workload)
         log(f"cpu baseline {time.perf_counter() - t0:.1f} s")
     if not args.no_batched:
-        line["batched"] = batched(torch, ljamd, args.steps)
+        line["batched"] = batched(torch, ljamd, args.steps, autotune=not args.no_autotune)
     print(json.dumps(line), flush=True)
 
 

@@ -94,7 +94,7 @@ typedef struct lj_params {
     int32_t refine;    /* force-grid cells per cell_edge along each axis: 0, 1, 2 or 4;
                           the candidate count keeps the cell_edge grid of the reference */
     int16_t block;     /* particles per workgroup: 0, 64, 128 or 256 */
-    int16_t depth;     /* LDS queue entries per particle: 0, 8, 16 or 32 */
+    int16_t depth;     /* LDS queue entries per particle: 0, 8, 16 or 32; 1 = no queue */
 } lj_params;
 
 /* Scalar results, written to DEVICE memory by the compute calls. */

@@ -78,8 +78,8 @@ int make_geom(lj_context* ctx, const lj_params* p, int64_t n, GridGeom* g, PairC
         return fail(ctx, LJ_ERR_INVALID_ARG, "refine must be 0, 1, 2 or 4 (got %d)", p->refine);
     if (p->block != 0 && p->block != 64 && p->block != 128 && p->block != 256)
         return fail(ctx, LJ_ERR_INVALID_ARG, "block must be 0, 64, 128 or 256 (got %d)", (int)p->block);
-    if (p->depth != 0 && p->depth != 8 && p->depth != 16 && p->depth != 32)
-        return fail(ctx, LJ_ERR_INVALID_ARG, "depth must be 0, 8, 16 or 32 (got %d)", (int)p->depth);
+    if (p->depth != 0 && p->depth != 1 && p->depth != 8 && p->depth != 16 && p->depth != 32)
+        return fail(ctx, LJ_ERR_INVALID_ARG, "depth must be 0, 1, 8, 16 or 32 (got %d)", (int)p->depth);
     c->tri_lo = tri_lo();
     c->e4 = 4.0 * (double)p->epsilon;
     c->e24 = 24.0 * (double)p->epsilon;

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __rest
 // wave may span several small systems: each lane reads its own system's constants.
 // Per-particle energy, virial and pair count go to arrays in the caller's order for
 // the per-system reduction.
-template <int PRED, int PREC>
+template <int PRED, int PREC, bool QUEUE>
 __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restrict__ pos,
                                                            const uint32_t* __restrict__ cell_of,
                                                            const uint32_t* __restrict__ off, uint32_t n,
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, true> acc;
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
-    const uint32_t np = stencil_pairs<PRED, PREC, true>(pos, off, k, active, pi, g, base, key - base, R, tr, c,
+    const uint32_t np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, R, tr, c,
                                                         queue + threadIdx.x, blockDim.x, depth, acc);
     if (!active) return;
     const uint32_t i = __float_as_uint(pi.w);
@@ -268,12 +268,17 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     {
         PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
         const unsigned nb = div_up(n, sh.block);
-        const size_t lds = (size_t)sh.depth * (size_t)sh.block * sizeof(uint32_t);
+        const size_t lds = sh.lds_bytes();
         if (n > 0) {
-#define LJ_BF(PRED, PREC)                                                                                         \
-    hipLaunchKernelGGL((k_batch_force<PRED, PREC>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of,               \
+#define LJ_BFQ(PRED, PREC, Q)                                                                                     \
+    hipLaunchKernelGGL((k_batch_force<PRED, PREC, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of,            \
                        (const uint32_t*)off, (uint32_t)n, dsys, dcbase, (uint32_t)nsys, sh.refine, (uint32_t)sh.depth, \
                        fx, fy, fz, e, ev, npair)
+#define LJ_BF(PRED, PREC)                          \
+    do {                                           \
+        if (sh.queued()) LJ_BFQ(PRED, PREC, true); \
+        else LJ_BFQ(PRED, PREC, false);            \
+    } while (0)
 #define LJ_BF_PREC(PRED)                                   \
     switch (p0.precision) {                                \
     case LJ_PREC_MIXED: LJ_BF(PRED, LJ_PREC_MIXED); break; \
@@ -289,6 +294,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
             }
 #undef LJ_BF_PREC
 #undef LJ_BF
+#undef LJ_BFQ
         }
     }
     {

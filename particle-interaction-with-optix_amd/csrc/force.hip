@@ -19,10 +19,10 @@ namespace lj {
 namespace {
 
 // One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues in
-// dynamic LDS (depth x block words).  Outputs go to the caller's order (pi.w holds the
+// dynamic LDS (depth x block words; none without QUEUE).  Outputs go to the caller's order (pi.w holds the
 // original index); per-block FP64 energy and virial partials are reduced later in a
 // fixed order.
-template <int PRED, int PREC, bool VIR>
+template <int PRED, int PREC, bool VIR, bool QUEUE>
 __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
                                                      const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int R,
                                                      Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ 
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR> acc;
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
-    const uint32_t np = stencil_pairs<PRED, PREC, VIR>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u, R, tr,
+    const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u, R, tr,
                                                        c, queue + threadIdx.x, blockDim.x, depth, acc);
     double esum = 0.0, vsum = 0.0;
     if (active) {
@@ -171,16 +171,22 @@ void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t 
                   const uint32_t* cell_of, const uint32_t* off, uint32_t n, const GridGeom& g, const Trim& tr,
                   const PairConst& c, float* fx, float* fy, float* fz, float* e, double* partials,
                   unsigned long long* npairs) {
-    const size_t lds = (size_t)sh.depth * (size_t)sh.block * sizeof(uint32_t);
-#define LJ_LAUNCH(PREC, VIR)                                                                                  \
-    hipLaunchKernelGGL((k_force<PRED, PREC, VIR>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
+    const size_t lds = sh.lds_bytes();
+#define LJ_LAUNCH_Q(PREC, VIR, Q)                                                                                \
+    hipLaunchKernelGGL((k_force<PRED, PREC, VIR, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
                        sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs)
+#define LJ_LAUNCH(PREC, VIR)                   \
+    do {                                       \
+        if (sh.queued()) LJ_LAUNCH_Q(PREC, VIR, true); \
+        else LJ_LAUNCH_Q(PREC, VIR, false);    \
+    } while (0)
     switch (prec) {
     case LJ_PREC_MIXED: if (vir) LJ_LAUNCH(LJ_PREC_MIXED, true); else LJ_LAUNCH(LJ_PREC_MIXED, false); break;
     case LJ_PREC_FP32: if (vir) LJ_LAUNCH(LJ_PREC_FP32, true); else LJ_LAUNCH(LJ_PREC_FP32, false); break;
     default: if (vir) LJ_LAUNCH(LJ_PREC_FP32_COMP, true); else LJ_LAUNCH(LJ_PREC_FP32_COMP, false); break;
     }
 #undef LJ_LAUNCH
+#undef LJ_LAUNCH_Q
 }
 
 }  // namespace
@@ -256,7 +262,10 @@ Trim make_trim(const lj_params* p, const GridGeom& g) {
         mag = std::fmax(mag, std::fabs((double)p->box_hi[d]));
         t.hi[d] = p->box_hi[d];
     }
-    t.margin = (float)(8.0 * 0x1p-24 * mag + (double)p->rc * 0x1p-14 + 8.0 * 0x1p-24 * (double)g.cw);
+    // 2^-19 mag bounds the rounding of x_i +- w, of v - lo and (divided by the cell
+    // edge) the <= 3 ulp gap between (v - lo) * inv_cw and (v - lo) / cw
+    t.margin = (float)(0x1p-19 * mag + (double)p->rc * 0x1p-14 + 8.0 * 0x1p-24 * (double)g.cw);
+    t.inv_cw = 1.0f / g.cw;
     t.on = p->predicate != LJ_PRED_CUDALJ;
     return t;
 }

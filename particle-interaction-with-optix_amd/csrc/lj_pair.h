@@ -111,7 +111,8 @@ __device__ inline T wave_sum(T v) {
 }
 
 // Blocks of up to 256 threads (a whole number of waves): deterministic tree sum of two
-// doubles; thread 0 holds the result.
+// doubles; thread 0 holds the result.  Every thread must call it; the closing barrier
+// lets a block call it again (its LDS slots are reused).
 __device__ inline void block_sum2(double& a, double& b) {
     __shared__ double sh[2][256 / kWaveSize];
     a = wave_sum(a);
@@ -130,8 +131,10 @@ __device__ inline void block_sum2(double& a, double& b) {
             b += sh[1][q];
         }
     }
+    __syncthreads();
 }
 
+// Sum over the block (thread 0 holds it), callable repeatedly like block_sum2.
 __device__ inline unsigned long long block_sum_u64(unsigned long long v) {
     __shared__ unsigned long long sh[256 / kWaveSize];
     v = wave_sum(v);
@@ -140,6 +143,7 @@ __device__ inline unsigned long long block_sum_u64(unsigned long long v) {
     unsigned long long t = 0;
     if (threadIdx.x == 0)
         for (unsigned q = 0; q < blockDim.x / kWaveSize; ++q) t += sh[q];
+    __syncthreads();
     return t;
 }
 

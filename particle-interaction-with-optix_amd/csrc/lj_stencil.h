@@ -13,21 +13,32 @@
 namespace lj {
 
 // Stencil trimming constants: a row of the stencil is skipped when its (y,z) distance
-// to particle i exceeds the cutoff, and the x-cells of a row are limited to
-// [x_i - w, x_i + w] with w = sqrt(rc^2 - gap^2).  Every bound is widened by `margin`,
-// which exceeds the FP32 rounding of a cell assignment and of x_i +- w, so a row or
-// cell is dropped only when no particle in it can pass the predicate.
+// to particle i exceeds the cutoff, and the x-cells of a row are limited to the cells
+// of [x_i - w, x_i + w] with w = sqrt(rc^2 - gap^2), found by multiplying with the
+// rounded reciprocal of the cell edge.  Every bound is widened by `margin`, which
+// exceeds the FP32 rounding of x_i +- w and the difference between that product and
+// the cell assignment's correctly rounded quotient (a few ulp of the cell coordinate),
+// so a row or cell is dropped only when no particle in it can pass the predicate.
 struct Trim {
     float r2;       // (rc (1 + 2^-16))^2, rounded up
     float margin;   // absolute widening of every bound
     float hi[3];    // box_hi: pruning applies to particles inside the box
+    float inv_cw;   // 1 / cell edge of the force grid, rounded
     int on;         // 0 for the CUDALJ mode (no cutoff)
 };
 
+// Cell of coordinate v by the reciprocal, clamped like axis_cell (trimming only).
+__device__ inline int32_t trim_cell(float v, float lo, float inv, float gmax) {
+    return (int32_t)fminf(fmaxf((v - lo) * inv, 0.0f), gmax);
+}
+
 // Kernel shape of a force launch: force-grid refinement K (stencil radius K cells of
-// edge cell_edge/K), particles per workgroup and LDS queue entries per particle.
+// edge cell_edge/K), particles per workgroup and LDS queue entries per particle
+// (depth 1: no queue, the pair arithmetic runs in the candidate loop).
 struct Shape {
     int refine, block, depth;
+    bool queued() const { return depth > 1; }
+    size_t lds_bytes() const { return queued() ? (size_t)depth * (size_t)block * sizeof(uint32_t) : 0; }
 };
 
 constexpr int kMaxBlock = 256;
@@ -83,15 +94,18 @@ __device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const
 // wave whenever any one lane accepts (a quarter of the lanes at the cutoff densities
 // of interest); from the queues it runs with most lanes busy.  Queues are emptied in
 // order, so a particle's sum keeps the stencil order of its partners whatever the
-// queue depth.  Every lane of the wave must call this (inactive lanes with
-// active = false), because the flushes are wave-wide.
-template <int PRED, int PREC, bool VIR>
+// queue depth.  With QUEUE = false the pair arithmetic runs inside the candidate loop
+// instead (same order, same bits; faster when nearly every lane accepts).  Every lane
+// of the wave must call this (inactive lanes with active = false), because the
+// flushes are wave-wide.
+template <int PRED, int PREC, bool VIR, bool QUEUE>
 __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
                                          bool active, const float4& pi, const GridGeom& g, uint32_t cbase,
                                          uint32_t key, int R, const Trim& tr, const PairConst& c,
                                          uint32_t* __restrict__ queue, uint32_t stride, unsigned depth,
                                          PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR>& acc) {
-    unsigned pc = 0;  // queued partners of this lane
+    unsigned pc = 0;          // queued partners of this lane
+    uint32_t* tail = queue;   // = queue + pc * stride
     uint32_t np = 0;
     const uint32_t kload = active ? k : 0;  // an in-bounds slot for idle loads
     auto flush = [&]() {
@@ -109,13 +123,25 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
             }
         }
         pc = 0;
+        tail = queue;
     };
     auto visit = [&](uint32_t t, const float4& pj) {
-        const float d2 = dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-        if (t == k || !accept<PRED>(c, d2)) return;
-        queue[pc * stride] = t;
-        ++pc;
+        const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const float d2 = dist2(dx, dy, dz);
+        // the cutoff predicates reject d2 = 0, hence the particle itself; the CUDALJ
+        // mode accepts every candidate and skips itself by slot
+        if constexpr (PRED == LJ_PRED_CUDALJ) {
+            if (t == k) return;
+        }
+        if (!accept<PRED>(c, d2)) return;
         ++np;
+        if constexpr (QUEUE) {
+            *tail = t;
+            tail += stride;
+            ++pc;
+        } else {
+            acc.add(c, pj, dx, dy, dz, d2);
+        }
     };
     int32_t cx = 0, cy = 0, cz = 0;
     if (active) cell_coords(g, key, cx, cy, cz);
@@ -135,22 +161,25 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
                 const float w2 = tr.r2 - gz * gz - gy * gy;
                 if (w2 < 0.f) continue;
                 const float w = __builtin_sqrtf(w2) + tr.margin;
-                xa = max(x0, axis_cell(pi.x - w, g.lo[0], g.cw, g.gmax[0]));
-                xb = min(x1, axis_cell(pi.x + w, g.lo[0], g.cw, g.gmax[0]));
+                xa = max(x0, trim_cell(pi.x - w, g.lo[0], tr.inv_cw, g.gmax[0]));
+                xb = min(x1, trim_cell(pi.x + w, g.lo[0], tr.inv_cw, g.gmax[0]));
             }
             const uint32_t row = cbase + (uint32_t)((zz * g.g[1] + yy) * g.g[0]);
             const uint32_t b = off[row + xa], en = off[row + xb + 1];
             for (uint32_t t = b; t < en; t += 2) {
-                if (__any(pc + 2 > depth)) flush();
+                if constexpr (QUEUE)
+                    if (__any(pc + 2 > depth)) flush();
+                // slot t + 1 <= n exists (pos has a spare entry); it counts only below en
                 const float4 p0 = pos[t];
-                const float4 p1 = pos[t + 1 < en ? t + 1 : t];
+                const float4 p1 = pos[t + 1];
                 visit(t, p0);
                 if (t + 1 < en) visit(t + 1, p1);
             }
         }
     }
     // rows end at different times in different lanes: flush while any queue holds pairs
-    while (__any(pc > 0)) flush();
+    if constexpr (QUEUE)
+        while (__any(pc > 0)) flush();
     return np;
 }
 

@@ -345,8 +345,8 @@ int run_lj(int argc, char** argv) {
     long long n = 100000;
     double rc = 0.04;
     std::string gen = "uniform", prec = "mixed";
-    int loops = 5, refine = 0;
-    bool lbvh = false;
+    int loops = 5, refine = 0, block = 0, depth = 0;
+    bool lbvh = false, tune = false;
     uint64_t seed = 1234;
     for (int i = 2; i < argc; ++i) {
         const std::string a = argv[i];
@@ -362,6 +362,9 @@ int run_lj(int argc, char** argv) {
         else if (a == "--gen") gen = next();
         else if (a == "--loops") loops = std::atoi(next());
         else if (a == "--refine") refine = std::atoi(next());
+        else if (a == "--block") block = std::atoi(next());
+        else if (a == "--depth") depth = std::atoi(next());
+        else if (a == "--autotune") tune = true;
         else if (a == "--precision") prec = next();
         else if (a == "--seed") seed = std::strtoull(next(), nullptr, 10);
         else if (a == "--lbvh") lbvh = true;
@@ -385,10 +388,18 @@ int run_lj(int argc, char** argv) {
     }
     lj_params p = base_params((float)rc, (float)(rc / 2.5), 1.0f, lo, hi, LJ_PRED_AABB);
     p.refine = refine;
+    p.block = (int16_t)block;
+    p.depth = (int16_t)depth;
     p.precision = prec == "fp32" ? LJ_PREC_FP32 : prec == "fp32_comp" ? LJ_PREC_FP32_COMP : LJ_PREC_MIXED;
     Device dev;
     dev.reserve((size_t)n);
     upload(dev, x, y, z);
+    if (tune && !lbvh) {
+        lj_tuning t;
+        CHECK_LJ(ctx, lj_autotune(ctx, dev.x, dev.y, dev.z, n, &p, 3, &t, nullptr));
+        std::printf("autotune: refine %d, block %d, depth %d (%d shapes; %.4f ms vs default %.4f ms)\n", t.refine,
+                    t.block, t.depth, t.n_tried, t.ms_best, t.ms_default);
+    }
     (void)timed(ctx, lbvh, dev, n, p, nullptr);  // warm-up (grows the workspace)
     double ti = 0, tc = 0;
     for (int l = 0; l < loops; ++l) {
@@ -412,7 +423,8 @@ void usage(const char* argv0) {
                  "       %s aabb [-rb|--runbench] [-gs|--gensurface] [-us|--usedsort] [-ns|--nbspheres N]\n"
                  "            [-bd|--boxdivisor D] [--max-boxdiv M]\n"
                  "       %s lj [--n N] [--rc RC] [--gen uniform|surface|clusters] [--loops L] [--refine K]\n"
-                 "            [--precision mixed|fp32|fp32_comp] [--lbvh] [--seed S]\n",
+                 "            [--block B] [--depth D] [--autotune] [--precision mixed|fp32|fp32_comp] [--lbvh]\n"
+                 "            [--seed S]\n",
                  argv0, argv0, argv0);
     std::exit(1);
 }

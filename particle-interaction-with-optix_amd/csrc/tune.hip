@@ -17,7 +17,7 @@ namespace {
 
 constexpr int kRefines[] = {1, 2, 4};
 constexpr int kBlocks[] = {64, 128, 256};
-constexpr int kDepths[] = {8, 16, 32};
+constexpr int kDepths[] = {1, 8, 16, 32};  // 1: no queue
 
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;

@@ -130,7 +130,7 @@ def test_kernel_shapes_same_results(engine, name):
     print(f"[shapes] {name}", flush=True)
     base = None
     for block in (64, 128, 256):
-        for depth in (8, 16, 32):
+        for depth in (1, 8, 16, 32):
             p.block, p.depth = block, depth
             f, e, res = _forces(engine, x, y, z, p)
             cur = (np.stack(f).view(np.uint32), e.view(np.uint32), int(res["n_pairs"]), int(res["n_candidates"]),
@@ -152,9 +152,9 @@ def test_autotune_picks_a_valid_shape(engine):
     dx, dy, dz = engine.to_device(x, y, z)
     print("[autotune] c1", flush=True)
     t = engine.autotune(dx, dy, dz, p, reps=2)
-    assert t["refine"] in (1, 2, 4) and t["block"] in (64, 128, 256) and t["depth"] in (8, 16, 32)
+    assert t["refine"] in (1, 2, 4) and t["block"] in (64, 128, 256) and t["depth"] in (1, 8, 16, 32)
     assert (p.refine, p.block, p.depth) == (t["refine"], t["block"], t["depth"])
-    assert t["n_tried"] == 27
+    assert t["n_tried"] == 36
     assert 0 < t["ms_best"] and 0 < t["ms_default"]
     ref = po.forces_f64(po.from_lj(p), x, y, z)
     f, e, res = _forces(engine, x, y, z, p)
@@ -170,7 +170,7 @@ def test_autotune_cudalj_keeps_the_27_cells(engine):
     dx, dy, dz = engine.to_device(x[:20000], y[:20000], z[:20000])
     t = engine.autotune(dx, dy, dz, q, reps=1)
     assert t["refine"] == 1 and q.refine == 1
-    assert t["n_tried"] == 9
+    assert t["n_tried"] == 12
 
 
 def test_trimming_at_cell_faces(engine):
