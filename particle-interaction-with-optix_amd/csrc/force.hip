@@ -238,7 +238,9 @@ int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int K, G
 // Shape of a call.  Refinement: an explicit p->refine wins; otherwise the half-edge
 // grid when the cell_edge grid averages >= 2 particles per cell and the refined grid
 // stays within 2^28 cells (the CUDALJ mode keeps the reference's 27 cells).  Block 256
-// and queue depth 16 unless p says otherwise.
+// and no queue unless p says otherwise (measured on MI355X at the C3 density: the
+// queued chain's extra LDS and dependent partner loads cost more than its lane
+// utilisation gains).
 Shape choose_shape(const lj_params* p, int64_t n, uint64_t ncells) {
     Shape sh;
     if (p->predicate == LJ_PRED_CUDALJ) sh.refine = 1;
@@ -248,7 +250,7 @@ Shape choose_shape(const lj_params* p, int64_t n, uint64_t ncells) {
         sh.refine = (ppc >= 2.0 && (double)ncells * 8.0 < (double)(1u << 28)) ? 2 : 1;
     }
     sh.block = p->block ? p->block : 256;
-    sh.depth = p->depth ? p->depth : 16;
+    sh.depth = p->depth ? p->depth : 1;
     return sh;
 }
 
