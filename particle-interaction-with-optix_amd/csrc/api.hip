@@ -176,7 +176,7 @@ static void release_all(lj_context* ctx) {
                       &b.cell_of, &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets})
         d->release();
     LbvhWork& l = ctx->lbvh;
-    for (DevBuf* d : {&l.node_box, &l.node_child, &l.parent, &l.flags})
+    for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags})
         d->release();
     BatchWork& t = ctx->batch;
     for (DevBuf* d : {&t.sys, &t.cell_base, &t.part_sys, &t.tiles, &t.partials, &t.results_tmp}) d->release();

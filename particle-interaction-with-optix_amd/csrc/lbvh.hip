@@ -12,11 +12,19 @@
 //     (ties keep index order, so the sorted (code, index) keys are unique);
 //  2. the hierarchy of Karras, "Maximizing Parallelism in the Construction of BVHs,
 //     Octrees, and k-d Trees", HPG 2012 (one thread per internal node);
-//  3. bottom-up bounds: each leaf walks to the root, the second child to arrive at a
-//     node (acquire/release counter) merges the two child boxes;
-//  4. one thread per particle (in Morton order) traverses with a stack, testing the
-//     query box [p - rc', p + rc'] against node point-bounds, then the canonical
-//     predicate and the pair arithmetic of the cell grid on each leaf it reaches.
+//  3. node bounds without synchronisation: an internal node covers a contiguous range
+//     of leaves (Morton order), so its bounds are the min/max over that range, read
+//     from three levels of aligned chunk bounds (256, 65536 and 2^24 leaves) plus
+//     the leaves at the range's ends.  (Merging child boxes bottom-up would need an
+//     acquire/release handshake per node across workgroups; at device scope on a
+//     multi-XCD part each one writes back and invalidates the L2.)
+//  4. one thread per particle (in Morton order) walks the tree without a stack: every
+//     node stores, in the spare lanes of its bounds, its left child and its rope (the
+//     node that follows its subtree in depth-first order), so a step descends on
+//     overlap and follows the rope otherwise.  Subtrees of at most kBucket leaves are
+//     buckets: one box test, then a scan of their leaves, which are contiguous.  The
+//     query box [p - rc', p + rc'] is tested against node point-bounds, then the
+//     canonical predicate and the pair arithmetic of the cell grid run on each leaf.
 // A node stores the bounds of its particle positions; testing the query box of
 // half-width rc' = rc (1 + 2^-18) against them is the point-in-AABB test of the
 // reference with the rounding of p +- rc moved to the safe side: every pair the
@@ -28,8 +36,6 @@
 namespace lj {
 
 namespace {
-
-constexpr int kStack = 64;
 
 struct Box {
     float lo[3], hi[3];
@@ -68,9 +74,18 @@ __device__ inline int delta(const uint32_t* __restrict__ code, int n, int i, int
     return __clz(a ^ b);
 }
 
-// Node numbering: internal nodes 0..n-2 (0 is the root), leaf k is node n-1+k.
+// Node numbering: internal nodes 0..n-2 (0 is the root), leaf k is node n-1+k (n <=
+// 2^30, so every id fits 31 bits).  A child or rope reference with bit 31 set is a
+// bucket: an internal node covering at most kBucket leaves, scanned instead of walked.
+constexpr int kBucket = 8;
+constexpr int kCollapsed = (int)0x80000000u;
+constexpr int kSentinel = 0x7fffffff;  // the rope of the last subtree: end of the walk
+
+// Karras' construction.  Every split position gamma in [0, n-2] belongs to exactly
+// one internal node, whose right child starts at leaf gamma + 1; that right child is
+// therefore the rope of every subtree that ends at leaf gamma: rope_by_end[gamma].
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ code, int n, int2* __restrict__ child,
-                                                int* __restrict__ parent) {
+                                                int2* __restrict__ range, int* __restrict__ rope_by_end) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (delta(code, n, i, i + 1) - delta(code, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -90,11 +105,13 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     } while (t > 1);
     const int gamma = i + s * d + (d < 0 ? d : 0);
     const int lo = i < j ? i : j, hi = i < j ? j : i;
-    const int left = (lo == gamma) ? (n - 1 + gamma) : gamma;
-    const int right = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+    int left = (lo == gamma) ? (n - 1 + gamma) : gamma;
+    int right = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+    if (left < n - 1 && gamma - lo + 1 <= kBucket) left |= kCollapsed;
+    if (right < n - 1 && hi - gamma <= kBucket) right |= kCollapsed;
     child[i] = make_int2(left, right);
-    parent[left] = i;
-    parent[right] = i;
+    range[i] = make_int2(lo, hi);
+    rope_by_end[gamma] = right;
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, uint32_t n, int v) {
@@ -102,33 +119,107 @@ __global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, uint32_t 
     if (i < n) p[i] = v;
 }
 
-// Bottom-up bounds.  Node boxes: box[node] = (lo.xyz, hi.xyz) as two float4.
-__global__ __launch_bounds__(256) void k_refit(const float4* __restrict__ leaf, int n, const int2* __restrict__ child,
-                                               const int* __restrict__ parent, float4* __restrict__ box,
-                                               int* __restrict__ arrived, int* __restrict__ overflow) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const float4 p = leaf[k];
-    int node = n - 1 + k;
-    box[2 * node] = make_float4(p.x, p.y, p.z, 0.f);
-    box[2 * node + 1] = make_float4(p.x, p.y, p.z, 0.f);
-    int par = parent[node];
-    for (int depth = 0; par >= 0; ++depth) {
-        if (depth >= 2 * kStack) {  // deeper than any well-formed tree: stop and flag
-            atomicOr(overflow, 2);
-            return;
-        }
-        // release our box, acquire the sibling's: the second arrival proceeds
-        const int prev = __hip_atomic_fetch_add(&arrived[par], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;
-        const int2 c = child[par];
-        const float4 a0 = box[2 * c.x], a1 = box[2 * c.x + 1];
-        const float4 b0 = box[2 * c.y], b1 = box[2 * c.y + 1];
-        box[2 * par] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.f);
-        box[2 * par + 1] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.f);
-        node = par;
-        par = parent[node];
+constexpr int kChunkBits = 8;  // a chunk of level l + 1 holds 256 chunks of level l
+constexpr int kLevels = 3;     // chunks of 2^8, 2^16 and 2^24 leaves
+
+struct Bounds {
+    float lo[3], hi[3];
+    __device__ void empty() {
+        lo[0] = lo[1] = lo[2] = INFINITY;
+        hi[0] = hi[1] = hi[2] = -INFINITY;
     }
+    __device__ void add(float4 l, float4 h) {
+        lo[0] = fminf(lo[0], l.x);
+        lo[1] = fminf(lo[1], l.y);
+        lo[2] = fminf(lo[2], l.z);
+        hi[0] = fmaxf(hi[0], h.x);
+        hi[1] = fmaxf(hi[1], h.y);
+        hi[2] = fmaxf(hi[2], h.z);
+    }
+};
+
+// Bounds of aligned chunks: chunk c of the output covers inputs [256 c, 256 c + 256)
+// (leaf positions when `leaves`, else (lo, hi) pairs of the level below).
+__global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__ in, bool leaves, uint32_t count,
+                                                      float4* __restrict__ out) {
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    Bounds b;
+    b.empty();
+    if (e < count) {
+        if (leaves) b.add(in[e], in[e]);
+        else b.add(in[2 * e], in[2 * e + 1]);
+    }
+#pragma unroll
+    for (int d = kWaveSize / 2; d > 0; d >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = fminf(b.lo[a], __shfl_xor(b.lo[a], d, kWaveSize));
+            b.hi[a] = fmaxf(b.hi[a], __shfl_xor(b.hi[a], d, kWaveSize));
+        }
+    __shared__ float sh[256 / kWaveSize][6];
+    const unsigned w = threadIdx.x / kWaveSize;
+    if (__lane_id() == 0)
+        for (int a = 0; a < 3; ++a) {
+            sh[w][a] = b.lo[a];
+            sh[w][3 + a] = b.hi[a];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned q = 1; q < 256 / kWaveSize; ++q)
+            for (int a = 0; a < 3; ++a) {
+                b.lo[a] = fminf(b.lo[a], sh[q][a]);
+                b.hi[a] = fmaxf(b.hi[a], sh[q][3 + a]);
+            }
+        out[2 * blockIdx.x] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.f);
+        out[2 * blockIdx.x + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.f);
+    }
+}
+
+// Bounds of internal node i: min/max of the leaf positions in its range, from the
+// largest aligned chunks inside the range and single leaves at its ends.  chunk[l] is
+// level l + 1 (chunks of 2^(8 (l + 1)) leaves); counts are the full chunks per level.
+// The .w lanes carry the walk: lo.w = the left child, hi.w = the rope (references as
+// int bits).
+__global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ leaf, int n,
+                                                     const int2* __restrict__ range, const int2* __restrict__ child,
+                                                     const int* __restrict__ rope_by_end, const float4* __restrict__ c1,
+                                                     const float4* __restrict__ c2, const float4* __restrict__ c3,
+                                                     float4* __restrict__ box) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int2 r = range[i];
+    uint32_t a = (uint32_t)r.x;
+    const uint32_t b = (uint32_t)r.y + 1u;  // exclusive, <= n
+    Bounds bb;
+    bb.empty();
+    // element e of level l: a leaf (l = -1) or chunk e of level l (2^(8 (l + 1)) leaves)
+    auto add = [&](int l, uint32_t e) {
+        if (l < 0) {
+            const float4 p = leaf[e];
+            bb.add(p, p);
+        } else {
+            const float4* c = l == 0 ? c1 : (l == 1 ? c2 : c3);
+            bb.add(c[2 * e], c[2 * e + 1]);
+        }
+    };
+    // Climb: while a whole chunk of the next level lies inside [a, b), consume the
+    // elements of the current level up to that chunk's start (a stays aligned to the
+    // current level's size).
+    int l = -1;
+    while (l + 1 < kLevels) {
+        const uint32_t size = 1u << (kChunkBits * (l + 1));
+        const uint32_t up = 1u << (kChunkBits * (l + 2));  // l + 2 <= kLevels: at most 2^24
+        const uint32_t start = (a + up - 1u) / up * up;     // a, b < 2^31: no overflow
+        if (start + up > b) break;
+        for (; a < start; a += size) add(l, a / size);
+        ++l;
+    }
+    // Descend: whole elements of each level from the largest down to single leaves.
+    for (; l >= -1; --l) {
+        const uint32_t size = 1u << (kChunkBits * (l + 1));
+        for (; a + size <= b; a += size) add(l, a / size);
+    }
+    box[2 * i] = make_float4(bb.lo[0], bb.lo[1], bb.lo[2], __int_as_float(child[i].x));
+    box[2 * i + 1] = make_float4(bb.hi[0], bb.hi[1], bb.hi[2], __int_as_float(rope_by_end[r.y]));
 }
 
 // Does the query box [p - h, p + h] meet node bounds [lo, hi]?  Differences in FP32
@@ -139,46 +230,50 @@ __device__ inline bool overlaps(float4 p, float h, float4 lo, float4 hi) {
 }
 
 // Visits every leaf whose position lies within the query box of leaf k, calling
-// f(slot, leaf position) for each one except k itself.  Returns false on stack overflow
-// (the tree is deeper than kStack levels, impossible for 30-bit codes plus 32-bit
-// indices, but checked).
+// f(slot, leaf position) for each one except k itself.  Depth-first, without a stack:
+// on overlap go to the left child, otherwise (and after a leaf or bucket) follow the
+// rope.  Returns false if the walk exceeds 2n steps (impossible for a well-formed
+// tree, but checked).
 template <class F>
-__device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __restrict__ child,
-                                const float4* __restrict__ box, int n, int k, float4 p, float h, unsigned long long& tests,
-                                F&& f) {
+__device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __restrict__ range,
+                                const float4* __restrict__ box, const int* __restrict__ rope_by_end, int n, int k,
+                                float4 p, float h, unsigned long long& tests, F&& f) {
     if (n == 1) return true;
-    int stack[kStack];
-    int sp = 0;
-    stack[sp++] = 0;
-    int visits = 0;
-    while (sp > 0) {
-        if (++visits > n) return false;  // a well-formed tree has n - 1 internal nodes
-        const int node = stack[--sp];
-        const int2 c = child[node];
-        const int kids[2] = {c.x, c.y};
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int ch = kids[q];
-            if (ch >= n - 1) {
-                const int slot = ch - (n - 1);
-                const float4 pj = leaf[slot];
-                if ((pj.x - p.x) <= h && (p.x - pj.x) <= h && (pj.y - p.y) <= h && (p.y - pj.y) <= h &&
-                    (pj.z - p.z) <= h && (p.z - pj.z) <= h) {
-                    ++tests;
-                    if (slot != k) f(slot, pj);
-                }
-            } else if (overlaps(p, h, box[2 * ch], box[2 * ch + 1])) {
-                if (sp >= kStack) return false;
-                stack[sp++] = ch;
+    auto leaf_test = [&](int slot) {
+        const float4 pj = leaf[slot];
+        if ((pj.x - p.x) <= h && (p.x - pj.x) <= h && (pj.y - p.y) <= h && (p.y - pj.y) <= h &&
+            (pj.z - p.z) <= h && (p.z - pj.z) <= h) {
+            ++tests;
+            if (slot != k) f(slot, pj);
+        }
+    };
+    int node = n <= kBucket ? (0 | kCollapsed) : 0;  // a small tree is one bucket
+    const uint32_t limit = 2u * (uint32_t)n;
+    for (uint32_t steps = 0; node != kSentinel; ++steps) {
+        if (steps > limit) return false;
+        if (node < 0) {  // bucket
+            const int b = node & 0x7fffffff;
+            const float4 lo = box[2 * b], hi = box[2 * b + 1];
+            if (overlaps(p, h, lo, hi)) {
+                const int2 r = range[b];
+                for (int slot = r.x; slot <= r.y; ++slot) leaf_test(slot);
             }
+            node = __float_as_int(hi.w);
+        } else if (node >= n - 1) {  // a leaf that is not in a bucket
+            const int slot = node - (n - 1);
+            leaf_test(slot);
+            node = rope_by_end[slot];
+        } else {
+            const float4 lo = box[2 * node], hi = box[2 * node + 1];
+            node = overlaps(p, h, lo, hi) ? __float_as_int(lo.w) : __float_as_int(hi.w);
         }
     }
     return true;
 }
 
 template <int PRED, int PREC, bool VIR>
-__global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ leaf, const int2* __restrict__ child,
-                                                    const float4* __restrict__ box, int n, float h, PairConst c,
+__global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ leaf, const int2* __restrict__ range,
+                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
                                                     float* __restrict__ fx, float* __restrict__ fy,
                                                     float* __restrict__ fz, float* __restrict__ e,
                                                     double* __restrict__ partials, lj_result* __restrict__ res,
@@ -190,7 +285,7 @@ __global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ l
     if (k < n) {
         const float4 pi = leaf[k];
         acc.set_i(pi);
-        const bool ok = traverse(leaf, child, box, n, k, pi, h, tests, [&](int, float4 pj) {
+        const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, [&](int, float4 pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float d2 = dist2(dx, dy, dz);
             if (!accept<PRED>(c, d2)) return;
@@ -219,15 +314,15 @@ __global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ l
 }
 
 template <int PRED>
-__global__ __launch_bounds__(256) void k_lbvh_count(const float4* __restrict__ leaf, const int2* __restrict__ child,
-                                                    const float4* __restrict__ box, int n, float h, PairConst c,
+__global__ __launch_bounds__(256) void k_lbvh_count(const float4* __restrict__ leaf, const int2* __restrict__ range,
+                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
                                                     uint32_t* __restrict__ rowcnt, int* __restrict__ overflow) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float4 pi = leaf[k];
     uint32_t cnt = 0;
     unsigned long long tests = 0;
-    const bool ok = traverse(leaf, child, box, n, k, pi, h, tests, [&](int, float4 pj) {
+    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, [&](int, float4 pj) {
         if (accept<PRED>(c, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z))) ++cnt;
     });
     if (!ok) atomicOr(overflow, 1);
@@ -235,8 +330,8 @@ __global__ __launch_bounds__(256) void k_lbvh_count(const float4* __restrict__ l
 }
 
 template <int PRED>
-__global__ __launch_bounds__(256) void k_lbvh_fill(const float4* __restrict__ leaf, const int2* __restrict__ child,
-                                                   const float4* __restrict__ box, int n, float h, PairConst c,
+__global__ __launch_bounds__(256) void k_lbvh_fill(const float4* __restrict__ leaf, const int2* __restrict__ range,
+                                                   const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
                                                    const unsigned long long* __restrict__ rows,
                                                    uint32_t* __restrict__ cols, int* __restrict__ overflow) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -246,7 +341,7 @@ __global__ __launch_bounds__(256) void k_lbvh_fill(const float4* __restrict__ le
     const unsigned long long r0 = rows[i], r1 = rows[i + 1];
     unsigned long long w = r0;
     unsigned long long tests = 0;
-    const bool ok = traverse(leaf, child, box, n, k, pi, h, tests, [&](int, float4 pj) {
+    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, [&](int, float4 pj) {
         if (w < r1 && accept<PRED>(c, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z))) cols[w++] = __float_as_uint(pj.w);
     });
     if (!ok) atomicOr(overflow, 1);
@@ -263,8 +358,8 @@ __global__ __launch_bounds__(256) void k_lbvh_fill(const float4* __restrict__ le
 
 }  // namespace
 
-// The overflow flag lives after the n arrival counters in ctx->lbvh.flags.
-static int* overflow_flag(lj_context* ctx, int64_t n) { return ctx->lbvh.flags.as<int>() + n + 1; }
+// The traversal's overflow flag.
+static int* overflow_flag(lj_context* ctx, int64_t) { return ctx->lbvh.flags.as<int>(); }
 
 // Builds the tree over n particles into ctx->lbvh.  h is the query half-width.
 static int lbvh_build(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const lj_params* p,
@@ -273,9 +368,12 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     BinWork& B = ctx->bin;
     const size_t nn = (size_t)(n > 0 ? n : 1);
     LJ_HIP(ctx, L.node_child.grow(nn * sizeof(int2)));
-    LJ_HIP(ctx, L.parent.grow(2 * nn * sizeof(int)));
-    LJ_HIP(ctx, L.flags.grow(nn * sizeof(int) + 64));
-    LJ_HIP(ctx, L.node_box.grow(2 * nn * 2 * sizeof(float4)));
+    LJ_HIP(ctx, L.range.grow(nn * sizeof(int2)));
+    LJ_HIP(ctx, L.rope.grow(nn * sizeof(int)));
+    LJ_HIP(ctx, L.flags.grow(64));
+    LJ_HIP(ctx, L.node_box.grow(nn * 2 * sizeof(float4)));
+    // chunk bounds of the three levels: ceil(n/256) + ceil(n/2^16) + ceil(n/2^24) pairs
+    LJ_HIP(ctx, L.chunks.grow((nn / 256 + nn / 65536 + nn / (1u << 24) + 3) * 2 * sizeof(float4)));
     MortonKeyOp op;
     for (int d = 0; d < 3; ++d) {
         op.lo[d] = p->box_lo[d];
@@ -284,28 +382,46 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     // the sorted payload is the leaf array: positions in Morton order, w = original index
     LJ_TRY(sort_particles(ctx, x, y, z, n, op, 30, s));
     if (n == 0) return LJ_OK;
-    const unsigned nb = div_up(n, 256);
     const uint32_t* code = B.sorted_keys();
-    int* parent = L.parent.as<int>();
-    int* arrived = L.flags.as<int>();
-    hipLaunchKernelGGL(k_fill_i32, dim3(div_up(2 * n, 256)), dim3(256), 0, s, parent, (uint32_t)(2 * n), -1);
-    hipLaunchKernelGGL(k_fill_i32, dim3(div_up(n + 1, 256)), dim3(256), 0, s, arrived, (uint32_t)n + 1, 0);
+    int2* range = L.range.as<int2>();
+    int* rope = L.rope.as<int>();
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, rope + (n - 1), 1u, kSentinel);
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, s, code, (int)n, L.node_child.as<int2>(),
-                           parent);
+                           range, rope);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, overflow_flag(ctx, n), 1u, 0);
-    hipLaunchKernelGGL(k_refit, dim3(nb), dim3(256), 0, s, (const float4*)B.pos.as<float4>(), (int)n,
-                       (const int2*)L.node_child.as<int2>(), (const int*)parent, L.node_box.as<float4>(), arrived,
-                       overflow_flag(ctx, n));
+    // chunk bounds, level by level (level l + 1 from level l; level 1 from the leaves)
+    const float4* leaves = B.pos.as<float4>();
+    float4* lvl[3];
+    {
+        uint64_t cnt = (uint64_t)n, off = 0;
+        const float4* in = leaves;
+        for (int l = 0; l < 3; ++l) {
+            const uint64_t chunks = div_up(cnt, 256);
+            lvl[l] = L.chunks.as<float4>() + 2 * off;
+            hipLaunchKernelGGL(k_chunk_bounds, dim3((unsigned)chunks), dim3(256), 0, s, in, l == 0, (uint32_t)cnt,
+                               lvl[l]);
+            in = lvl[l];
+            cnt = chunks;
+            off += chunks;
+        }
+    }
+    if (n > 1)
+        hipLaunchKernelGGL(k_node_bounds, dim3(div_up(n - 1, 256)), dim3(256), 0, s, leaves, (int)n,
+                           (const int2*)range, (const int2*)L.node_child.as<int2>(), (const int*)rope,
+                           (const float4*)lvl[0], (const float4*)lvl[1], (const float4*)lvl[2],
+                           L.node_box.as<float4>());
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }
 
 static float query_half_width(const lj_params* p) { return p->rc * (1.0f + 0x1p-18f); }
 
-static int check_lbvh_params(lj_context* ctx, const lj_params* p) {
+static int check_lbvh_params(lj_context* ctx, const lj_params* p, int64_t n) {
     if (p->predicate == LJ_PRED_CUDALJ)
         return fail(ctx, LJ_ERR_INVALID_ARG, "the CUDALJ predicate has no cutoff; the LBVH needs one");
+    if (n > (int64_t)1 << 30)
+        return fail(ctx, LJ_ERR_INVALID_ARG, "the LBVH takes at most 2^30 particles (node ids are 31-bit)");
     return LJ_OK;
 }
 
@@ -321,7 +437,7 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     GridGeom g;
     PairConst c;
     LJ_TRY(make_geom(ctx, p, n, &g, &c));
-    LJ_TRY(check_lbvh_params(ctx, p));
+    LJ_TRY(check_lbvh_params(ctx, p, n));
     if (n > 0 && (!x || !y || !z || !fx || !fy || !fz))
         return fail(ctx, LJ_ERR_INVALID_ARG, "positions and forces must not be NULL");
     LJ_HIP(ctx, hipSetDevice(ctx->device));
@@ -342,11 +458,12 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     int* ovf = overflow_flag(ctx, n);
     const float h = query_half_width(p);
     const float4* leaf = ctx->bin.pos.as<float4>();
-    const int2* child = ctx->lbvh.node_child.as<int2>();
+    const int2* range = ctx->lbvh.range.as<int2>();
+    const int* rope = ctx->lbvh.rope.as<int>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
 #define LJ_LB(PRED, PREC, VIR)                                                                                        \
-    hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(256), 0, s, leaf, child, box, (int)n, h, c, fx, \
+    hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(256), 0, s, leaf, range, rope, box, (int)n, h, c, fx, \
                        fy, fz, e, partials, res, ovf)
 #define LJ_LB_PREC(PRED)                                                                                    \
     switch (p->precision) {                                                                                 \
@@ -363,7 +480,7 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
         case LJ_PRED_TRIANGLE: LJ_LB_PREC(LJ_PRED_TRIANGLE); break;
         }
     } else {
-        hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true>), dim3(1), dim3(256), 0, s, leaf, child, box,
+        hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true>), dim3(1), dim3(256), 0, s, leaf, range, rope, box,
                            0, h, c, fx, fy, fz, e, partials, res, ovf);
     }
 #undef LJ_LB_PREC
@@ -380,7 +497,7 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     GridGeom g;
     PairConst c;
     LJ_TRY(make_geom(ctx, p, n, &g, &c));
-    LJ_TRY(check_lbvh_params(ctx, p));
+    LJ_TRY(check_lbvh_params(ctx, p, n));
     if (!row_offsets || (n > 0 && (!x || !y || !z)) || (capacity > 0 && !cols))
         return fail(ctx, LJ_ERR_INVALID_ARG, "NULL pointer argument");
     LJ_HIP(ctx, hipSetDevice(ctx->device));
@@ -392,11 +509,12 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     int* ovf = overflow_flag(ctx, n);
     const float h = query_half_width(p);
     const float4* leaf = ctx->bin.pos.as<float4>();
-    const int2* child = ctx->lbvh.node_child.as<int2>();
+    const int2* range = ctx->lbvh.range.as<int2>();
+    const int* rope = ctx->lbvh.rope.as<int>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
     if (n > 0) {
         switch (p->predicate) {
-#define LJ_C(P) case P: hipLaunchKernelGGL(k_lbvh_count<P>, dim3(nb), dim3(256), 0, s, leaf, child, box, (int)n, h, c, rowcnt, ovf); break;
+#define LJ_C(P) case P: hipLaunchKernelGGL(k_lbvh_count<P>, dim3(nb), dim3(256), 0, s, leaf, range, rope, box, (int)n, h, c, rowcnt, ovf); break;
             LJ_C(LJ_PRED_AABB) LJ_C(LJ_PRED_CUSTOM) LJ_C(LJ_PRED_SPHERE) LJ_C(LJ_PRED_TRIANGLE)
 #undef LJ_C
         }
@@ -407,12 +525,12 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     LJ_HIP(ctx, hipMemcpyAsync(&total, row_offsets + n, sizeof total, hipMemcpyDeviceToHost, s));
     LJ_HIP(ctx, hipMemcpyAsync(&overflow, ovf, sizeof overflow, hipMemcpyDeviceToHost, s));
     LJ_HIP(ctx, hipStreamSynchronize(s));
-    if (overflow) return fail(ctx, LJ_ERR_OVERFLOW, "LBVH deeper than the traversal stack (flag %d)", overflow);
+    if (overflow) return fail(ctx, LJ_ERR_OVERFLOW, "LBVH walk exceeded 2n steps: malformed tree (flag %d)", overflow);
     if (needed) *needed = total;
     if (total > capacity) return fail(ctx, LJ_ERR_CAPACITY, "pair list needs %llu entries, capacity is %llu", total, capacity);
     if (total == 0) return LJ_OK;
     switch (p->predicate) {
-#define LJ_F(P) case P: hipLaunchKernelGGL(k_lbvh_fill<P>, dim3(nb), dim3(256), 0, s, leaf, child, box, (int)n, h, c, (const unsigned long long*)row_offsets, cols, ovf); break;
+#define LJ_F(P) case P: hipLaunchKernelGGL(k_lbvh_fill<P>, dim3(nb), dim3(256), 0, s, leaf, range, rope, box, (int)n, h, c, (const unsigned long long*)row_offsets, cols, ovf); break;
         LJ_F(LJ_PRED_AABB) LJ_F(LJ_PRED_CUSTOM) LJ_F(LJ_PRED_SPHERE) LJ_F(LJ_PRED_TRIANGLE)
 #undef LJ_F
     }

@@ -70,10 +70,12 @@ struct BinWork {
 // The LBVH keeps its leaves (positions in Morton order) in BinWork::pos and its
 // Morton codes in BinWork::keys.
 struct LbvhWork {
-    DevBuf node_box;      // float4 x2 per node (lo, hi)
+    DevBuf node_box;      // float4 x2 per internal node (lo, hi)
     DevBuf node_child;    // int2 per internal node
-    DevBuf parent;        // int per node
-    DevBuf flags;         // refit arrival counters, then the overflow flag
+    DevBuf range;         // int2 per internal node: first and last leaf it covers
+    DevBuf chunks;        // float4 x2 bounds of aligned leaf chunks (256, 65536, 2^24 leaves)
+    DevBuf rope;          // int per leaf: the node after every subtree ending at that leaf
+    DevBuf flags;         // the traversal overflow flag
 };
 
 struct BatchWork {
