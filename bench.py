@@ -162,7 +162,8 @@ def batched(torch, ljamd, steps, autotune=True):
         "systems_per_s_batched": nsys / (best / 1e3),
         "systems_per_s_serial": nsys / (t_serial / 1e3),
         "speedup": t_serial / best,
-        "shape": {"refine": systems[0].params.refine, "block": systems[0].params.block, "depth": systems[0].params.depth},
+        "shape": {"refine": systems[0].params.refine, "refine_x": systems[0].params.refine_x,
+                  "block": systems[0].params.block, "depth": systems[0].params.depth},
         "autotune": tuning,
     }
 
@@ -178,7 +179,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="use the default kernel shape")
-    ap.add_argument("--shape", default=None, metavar="REFINE,BLOCK,DEPTH",
+    ap.add_argument("--shape", default=None, metavar="REFINE,REFINE_X,BLOCK,DEPTH",
                     help="fixed kernel shape for the single-system steps instead of autotuning")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference kernel for PMC byte counters)")
@@ -201,7 +202,7 @@ def main():
     n = w.x.size
     tuning = None
     if args.shape:
-        w.params.refine, w.params.block, w.params.depth = (int(v) for v in args.shape.split(","))
+        w.params.refine, w.params.refine_x, w.params.block, w.params.depth = (int(v) for v in args.shape.split(","))
     elif not args.no_autotune and not args.lbvh:
         t0 = time.perf_counter()
         tuning = eng.autotune(dx, dy, dz, w.params, reps=3)
@@ -246,7 +247,8 @@ def main():
             "ms_force": ms_force,
             "pairs_per_s_force_kernel": pairs / (ms_force / 1e3),
             "e_total": float(res["e_total"]),
-            "shape": {"refine": w.params.refine, "block": w.params.block, "depth": w.params.depth},
+            "shape": {"refine": w.params.refine, "refine_x": w.params.refine_x, "block": w.params.block,
+                      "depth": w.params.depth},
             "autotune": tuning,
         },
         "roofline": {

@@ -91,8 +91,10 @@ typedef struct lj_params {
     /* Kernel shape (0 = the library's default; lj_autotune picks measured values).
      * The shape changes speed only: pair counts are identical for every shape, and
      * forces and energies agree to the rounding of the final FP32 outputs. */
-    int32_t refine;    /* force-grid cells per cell_edge along each axis: 0, 1, 2 or 4;
+    int16_t refine;    /* force-grid cells per cell_edge along y and z: 0, 1, 2 or 4;
                           the candidate count keeps the cell_edge grid of the reference */
+    int16_t refine_x;  /* force-grid cells per cell_edge along x: 0 (= refine), 1, 2, 4, 8
+                          or 16, not below refine */
     int16_t block;     /* particles per workgroup: 0, 64, 128 or 256 */
     int16_t depth;     /* LDS queue entries per particle: 0, 8, 16 or 32; 1 = no queue */
 } lj_params;
@@ -215,16 +217,19 @@ int lj_profile_read(lj_context* ctx, double ms[3], unsigned long long calls[3]);
 /* Autotuner.  Reference: none -- the reference fixes its launch shape (one thread
  * per particle in blocks of DefaultNbThreads = 128, cudaLJ.cu:519-520, 647-650) and
  * its cell edge (= cutoff, cudaLJ.cu:525-531).  lj_autotune times lj_compute on the
- * given device positions for every valid (refine, block, depth), `reps` calls each
- * after one untimed call, and writes the fastest shape into p->refine, p->block and
- * p->depth (no other field changes).  Outputs of the timed calls go to the context's
+ * given device positions, `reps` calls per shape after one untimed call: first every
+ * valid grid (refine, refine_x) with block 256 and no queue, then every (block, depth)
+ * on the fastest grid.  It writes the fastest shape into p->refine, p->refine_x,
+ * p->block and p->depth (no other field changes).  Outputs of the timed calls go to the context's
  * workspace.  It synchronises `stream` and records hipEvents, so it cannot be
  * captured in a graph: tune once per workload, then pass the tuned params. */
 typedef struct lj_tuning {
-    int32_t refine, block, depth; /* chosen shape (also written to the params) */
+    int32_t refine, refine_x;     /* chosen grid (also written to the params) */
+    int32_t block, depth;         /* chosen workgroup size and queue depth */
     int32_t n_tried;              /* shapes timed */
     float ms_best;                /* mean time of one call with the chosen shape */
     float ms_default;             /* mean time of one call with the default shape */
+    int32_t reserved;
 } lj_tuning;
 int lj_autotune(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
                 lj_params* p, int32_t reps, lj_tuning* out, void* stream);

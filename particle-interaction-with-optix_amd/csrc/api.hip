@@ -47,7 +47,7 @@ static int grid_of(lj_context* ctx, const lj_params* p, GridGeom* g) {
         nc *= (uint64_t)gd;
     }
     if (nc > 0x7fffffffull) return fail(ctx, LJ_ERR_OVERFLOW, "cell count %llu exceeds 2^31-1", (unsigned long long)nc);
-    g->cw = cw;
+    g->cw[0] = g->cw[1] = g->cw[2] = cw;
     g->ncells = (uint32_t)nc;
     return LJ_OK;
 }
@@ -75,7 +75,13 @@ int make_geom(lj_context* ctx, const lj_params* p, int64_t n, GridGeom* g, PairC
     if (p->precision < LJ_PREC_MIXED || p->precision > LJ_PREC_FP32_COMP)
         return fail(ctx, LJ_ERR_INVALID_ARG, "unknown precision %d", p->precision);
     if (p->refine != 0 && p->refine != 1 && p->refine != 2 && p->refine != 4)
-        return fail(ctx, LJ_ERR_INVALID_ARG, "refine must be 0, 1, 2 or 4 (got %d)", p->refine);
+        return fail(ctx, LJ_ERR_INVALID_ARG, "refine must be 0, 1, 2 or 4 (got %d)", (int)p->refine);
+    if (p->refine_x != 0 && p->refine_x != 1 && p->refine_x != 2 && p->refine_x != 4 && p->refine_x != 8 &&
+        p->refine_x != 16)
+        return fail(ctx, LJ_ERR_INVALID_ARG, "refine_x must be 0, 1, 2, 4, 8 or 16 (got %d)", (int)p->refine_x);
+    if (p->refine_x != 0 && p->refine != 0 && p->refine_x < p->refine)
+        return fail(ctx, LJ_ERR_INVALID_ARG, "refine_x (%d) must not be below refine (%d)", (int)p->refine_x,
+                    (int)p->refine);
     if (p->block != 0 && p->block != 64 && p->block != 128 && p->block != 256)
         return fail(ctx, LJ_ERR_INVALID_ARG, "block must be 0, 64, 128 or 256 (got %d)", (int)p->block);
     if (p->depth != 0 && p->depth != 1 && p->depth != 8 && p->depth != 16 && p->depth != 32)

@@ -52,12 +52,12 @@ struct BatchKeyOp {
 // Particle counts of the cell_edge grids from the force grids' offsets (refine > 1).
 __global__ __launch_bounds__(256) void k_batch_coarse_counts(const uint32_t* __restrict__ foff, uint32_t nccells,
                                                              const SysDesc* __restrict__ sys,
-                                                             const uint32_t* __restrict__ ccbase, uint32_t nsys, int K,
-                                                             uint32_t* __restrict__ ccount) {
+                                                             const uint32_t* __restrict__ ccbase, uint32_t nsys, int Kx,
+                                                             int K, uint32_t* __restrict__ ccount) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nccells) return;
     const SysDesc& d = sys[find_sys(ccbase, nsys, c)];
-    ccount[c] = coarse_count(foff + d.cell_base, d.g, d.cg, K, c - d.ccell_base);
+    ccount[c] = coarse_count(foff + d.cell_base, d.g, d.cg, Kx, K, c - d.ccell_base);
 }
 
 // 27-cell candidates per system on the cell_edge grids (offsets coff over the
@@ -99,7 +99,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
                                                            const uint32_t* __restrict__ cell_of,
                                                            const uint32_t* __restrict__ off, uint32_t n,
                                                            const SysDesc* __restrict__ sys,
-                                                           const uint32_t* __restrict__ cbase, uint32_t nsys, int R,
+                                                           const uint32_t* __restrict__ cbase, uint32_t nsys, int Rx,
+                                                           int R,
                                                            uint32_t depth, float* __restrict__ fx, float* __restrict__ fy,
                                                            float* __restrict__ fz, float* __restrict__ e,
                                                            double2* __restrict__ ev, uint32_t* __restrict__ npair) {
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, true> acc;
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
-    const uint32_t np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, R, tr, c,
+    const uint32_t np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, Rx, R, tr, c,
                                                         queue + threadIdx.x, blockDim.x, depth, acc);
     if (!active) return;
     const uint32_t i = __float_as_uint(pi.w);
@@ -173,8 +174,8 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         LJ_TRY(make_geom(ctx, &ps, sys[s].n, &g, &c));
         if (ps.predicate != p0.predicate || ps.precision != p0.precision)
             return fail(ctx, LJ_ERR_INVALID_ARG, "all systems of a batch must share predicate and precision");
-        if (ps.refine != p0.refine || ps.block != p0.block || ps.depth != p0.depth)
-            return fail(ctx, LJ_ERR_INVALID_ARG, "all systems of a batch must share refine, block and depth");
+        if (ps.refine != p0.refine || ps.refine_x != p0.refine_x || ps.block != p0.block || ps.depth != p0.depth)
+            return fail(ctx, LJ_ERR_INVALID_ARG, "all systems of a batch must share refine, refine_x, block and depth");
         if (sys[s].first != (int64_t)ntot)
             return fail(ctx, LJ_ERR_INVALID_ARG, "system %d starts at %lld; systems must be packed back to back from 0",
                         s, (long long)sys[s].first);
@@ -195,7 +196,8 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     const Shape sh = choose_shape(&p0, (int64_t)ntot, cctot);
     for (int32_t s = 0; s < nsys; ++s) {
         SysDesc& d = desc[(size_t)s];
-        if (sh.refine > 1) LJ_TRY(refine_geom(ctx, d.cg, &sys[s].params, sh.refine, &d.g));
+        if (sh.refine > 1 || sh.refine_x > 1)
+            LJ_TRY(refine_geom(ctx, d.cg, &sys[s].params, sh.refine_x, sh.refine, &d.g));
         d.tr = make_trim(&sys[s].params, d.g);
         d.cell_base = (uint32_t)ctot;
         ctot += d.g.ncells;
@@ -249,11 +251,12 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     {
         PhaseTimer to(ctx, LJ_PHASE_OTHER, s);
         const uint32_t* coff = off;
-        if (sh.refine > 1) {
+        if (sh.refine > 1 || sh.refine_x > 1) {
             LJ_HIP(ctx, B.ccounts.grow((size_t)nccells * 4));
             LJ_HIP(ctx, B.coffsets.grow(((size_t)nccells + 1) * 4));
             hipLaunchKernelGGL(k_batch_coarse_counts, dim3(div_up(nccells, 256)), dim3(256), 0, s, (const uint32_t*)off,
-                               nccells, dsys, dccbase, (uint32_t)nsys, sh.refine, B.ccounts.as<uint32_t>());
+                               nccells, dsys, dccbase, (uint32_t)nsys, sh.refine_x, sh.refine,
+                               B.ccounts.as<uint32_t>());
             uint32_t* co = B.coffsets.as<uint32_t>();
             LJ_TRY(scan_u32(ctx, B.ccounts.as<uint32_t>(), co, nccells, co + nccells, s));
             coff = co;
@@ -272,7 +275,8 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         if (n > 0) {
 #define LJ_BFQ(PRED, PREC, Q)                                                                                     \
     hipLaunchKernelGGL((k_batch_force<PRED, PREC, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of,            \
-                       (const uint32_t*)off, (uint32_t)n, dsys, dcbase, (uint32_t)nsys, sh.refine, (uint32_t)sh.depth, \
+                       (const uint32_t*)off, (uint32_t)n, dsys, dcbase, (uint32_t)nsys, sh.refine_x, sh.refine, \
+                       (uint32_t)sh.depth, \
                        fx, fy, fz, e, ev, npair)
 #define LJ_BF(PRED, PREC)                          \
     do {                                           \

@@ -24,7 +24,8 @@ namespace {
 // fixed order.
 template <int PRED, int PREC, bool VIR, bool QUEUE>
 __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
-                                                     const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int R,
+                                                     const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int Rx,
+                                                     int R,
                                                      Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
                                                      float* __restrict__ fy, float* __restrict__ fz,
                                                      float* __restrict__ e, double* __restrict__ partials,
@@ -36,7 +37,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ 
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR> acc;
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
-    const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u, R, tr,
+    const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u, Rx, R, tr,
                                                        c, queue + threadIdx.x, blockDim.x, depth, acc);
     double esum = 0.0, vsum = 0.0;
     if (active) {
@@ -58,11 +59,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(256) void k_coarse_counts(const uint32_t* __restrict__ foff, GridGeom fg, GridGeom cg, int K,
-                                                       uint32_t* __restrict__ ccount) {
+__global__ __launch_bounds__(256) void k_coarse_counts(const uint32_t* __restrict__ foff, GridGeom fg, GridGeom cg, int Kx,
+                                                       int K, uint32_t* __restrict__ ccount) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cg.ncells) return;
-    ccount[c] = coarse_count(foff, fg, cg, K, c);
+    ccount[c] = coarse_count(foff, fg, cg, Kx, K, c);
 }
 
 // FP64 reduction of the per-block partials in a fixed order: stage 1 sums chunks of
@@ -174,7 +175,7 @@ void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t 
     const size_t lds = sh.lds_bytes();
 #define LJ_LAUNCH_Q(PREC, VIR, Q)                                                                                \
     hipLaunchKernelGGL((k_force<PRED, PREC, VIR, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
-                       sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs)
+                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs)
 #define LJ_LAUNCH(PREC, VIR)                   \
     do {                                       \
         if (sh.queued()) LJ_LAUNCH_Q(PREC, VIR, true); \
@@ -217,14 +218,19 @@ static int result_slot(lj_context* ctx, lj_result* user, lj_result** out) {
     return LJ_OK;
 }
 
-// Force grid of edge cw/K (K = 2 or 4, exact), G' = (int)(L / (cw/K)).
-int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int K, GridGeom* f) {
+// Force grid of edge cw/Kx along x and cw/K along y and z (powers of two: exact),
+// G' = (int)(L / edge) per axis.
+int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int Kx, int K, GridGeom* f) {
     *f = g;
-    f->cw = g.cw / (float)K;
+    f->cw[0] = g.cw[0] / (float)Kx;
+    f->cw[1] = g.cw[1] / (float)K;
+    f->cw[2] = g.cw[2] / (float)K;
     uint64_t nc = 1;
     for (int d = 0; d < 3; ++d) {
         const float L = p->box_hi[d] - p->box_lo[d];
-        int32_t gd = (int32_t)(L / f->cw);
+        const float q = L / f->cw[d];
+        if (!(q < 16777216.0f)) return fail(ctx, LJ_ERR_OVERFLOW, "refined grid: more than 2^24 cells along one axis");
+        int32_t gd = (int32_t)q;
         if (gd < 1) gd = 1;
         f->g[d] = gd;
         f->gmax[d] = (float)(gd - 1);
@@ -243,11 +249,16 @@ int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int K, G
 // utilisation gains).
 Shape choose_shape(const lj_params* p, int64_t n, uint64_t ncells) {
     Shape sh;
-    if (p->predicate == LJ_PRED_CUDALJ) sh.refine = 1;
-    else if (p->refine == 1 || p->refine == 2 || p->refine == 4) sh.refine = p->refine;
-    else {
-        const double ppc = (double)n / (double)(ncells ? ncells : 1);
-        sh.refine = (ppc >= 2.0 && (double)ncells * 8.0 < (double)(1u << 28)) ? 2 : 1;
+    if (p->predicate == LJ_PRED_CUDALJ) {
+        sh.refine = sh.refine_x = 1;
+    } else {
+        if (p->refine == 1 || p->refine == 2 || p->refine == 4) sh.refine = p->refine;
+        else {
+            const double ppc = (double)n / (double)(ncells ? ncells : 1);
+            sh.refine = (ppc >= 2.0 && (double)ncells * 8.0 < (double)(1u << 28)) ? 2 : 1;
+        }
+        sh.refine_x = p->refine_x ? p->refine_x : sh.refine;
+        if (sh.refine_x < sh.refine) sh.refine_x = sh.refine;
     }
     sh.block = p->block ? p->block : 256;
     sh.depth = p->depth ? p->depth : 1;
@@ -266,8 +277,9 @@ Trim make_trim(const lj_params* p, const GridGeom& g) {
     }
     // 2^-19 mag bounds the rounding of x_i +- w, of v - lo and (divided by the cell
     // edge) the <= 3 ulp gap between (v - lo) * inv_cw and (v - lo) / cw
-    t.margin = (float)(0x1p-19 * mag + (double)p->rc * 0x1p-14 + 8.0 * 0x1p-24 * (double)g.cw);
-    t.inv_cw = 1.0f / g.cw;
+    const double cwmax = std::fmax((double)g.cw[0], std::fmax((double)g.cw[1], (double)g.cw[2]));
+    t.margin = (float)(0x1p-19 * mag + (double)p->rc * 0x1p-14 + 8.0 * 0x1p-24 * cwmax);
+    t.inv_cw = 1.0f / g.cw[0];
     t.on = p->predicate != LJ_PRED_CUDALJ;
     return t;
 }
@@ -276,9 +288,9 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
                      const GridGeom& g, const PairConst& c, float* fx, float* fy, float* fz, float* e,
                      lj_result* result, hipStream_t s) {
     const Shape sh = choose_shape(p, n, g.ncells);
-    const int refine = sh.refine;
+    const bool refined = sh.refine > 1 || sh.refine_x > 1;
     GridGeom fg = g;
-    if (refine > 1) LJ_TRY(refine_geom(ctx, g, p, refine, &fg));
+    if (refined) LJ_TRY(refine_geom(ctx, g, p, sh.refine_x, sh.refine, &fg));
     const Trim tr = make_trim(p, fg);
     {
         PhaseTimer t(ctx, LJ_PHASE_BIN, s);
@@ -288,7 +300,7 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
     LJ_TRY(result_slot(ctx, result, &res));
     {
         PhaseTimer t(ctx, LJ_PHASE_OTHER, s);
-        if (refine == 1) {
+        if (!refined) {
             LJ_TRY(count_candidates(ctx, g, &res->n_candidates, s));
         } else {
             // candidate count on the cell_edge grid (ComputeNbInteractions semantics)
@@ -296,7 +308,8 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
             LJ_HIP(ctx, w.ccounts.grow((size_t)g.ncells * 4));
             LJ_HIP(ctx, w.coffsets.grow(((size_t)g.ncells + 1) * 4));
             hipLaunchKernelGGL(k_coarse_counts, dim3(div_up(g.ncells, 256)), dim3(256), 0, s,
-                               (const uint32_t*)w.offsets.as<uint32_t>(), fg, g, refine, w.ccounts.as<uint32_t>());
+                               (const uint32_t*)w.offsets.as<uint32_t>(), fg, g, sh.refine_x, sh.refine,
+                               w.ccounts.as<uint32_t>());
             uint32_t* coff = w.coffsets.as<uint32_t>();
             LJ_TRY(scan_u32(ctx, w.ccounts.as<uint32_t>(), coff, g.ncells, coff + g.ncells, s));
             LJ_TRY(count_candidates_from(ctx, g, coff, &res->n_candidates, s));

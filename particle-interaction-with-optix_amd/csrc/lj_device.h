@@ -15,7 +15,7 @@ namespace lj {
 // Kernel-side view of lj_params with the derived constants precomputed on the host.
 struct GridGeom {
     float lo[3];
-    float cw;          // cell edge
+    float cw[3];       // cell edge per axis (equal on the cell_edge grid)
     int32_t g[3];      // cells per axis
     float gmax[3];     // (float)(g-1), the clamp of the cell coordinate
     uint32_t ncells;
@@ -46,9 +46,9 @@ __host__ __device__ inline int32_t axis_cell(float x, float lo, float cw, float 
 
 // CellCoordToIndex (cudaLJ.cu:127-134): ((z*Gy)+y)*Gx+x.
 __host__ __device__ inline uint32_t cell_key(const GridGeom& g, float x, float y, float z) {
-    const int32_t cx = axis_cell(x, g.lo[0], g.cw, g.gmax[0]);
-    const int32_t cy = axis_cell(y, g.lo[1], g.cw, g.gmax[1]);
-    const int32_t cz = axis_cell(z, g.lo[2], g.cw, g.gmax[2]);
+    const int32_t cx = axis_cell(x, g.lo[0], g.cw[0], g.gmax[0]);
+    const int32_t cy = axis_cell(y, g.lo[1], g.cw[1], g.gmax[1]);
+    const int32_t cz = axis_cell(z, g.lo[2], g.cw[2], g.gmax[2]);
     return (uint32_t)((cz * g.g[1] + cy) * g.g[0] + cx);
 }
 

@@ -23,7 +23,7 @@ struct Trim {
     float r2;       // (rc (1 + 2^-16))^2, rounded up
     float margin;   // absolute widening of every bound
     float hi[3];    // box_hi: pruning applies to particles inside the box
-    float inv_cw;   // 1 / cell edge of the force grid, rounded
+    float inv_cw;   // 1 / x cell edge of the force grid, rounded
     int on;         // 0 for the CUDALJ mode (no cutoff)
 };
 
@@ -32,11 +32,12 @@ __device__ inline int32_t trim_cell(float v, float lo, float inv, float gmax) {
     return (int32_t)fminf(fmaxf((v - lo) * inv, 0.0f), gmax);
 }
 
-// Kernel shape of a force launch: force-grid refinement K (stencil radius K cells of
-// edge cell_edge/K), particles per workgroup and LDS queue entries per particle
-// (depth 1: no queue, the pair arithmetic runs in the candidate loop).
+// Kernel shape of a force launch: force-grid refinement (cells of edge cell_edge/Kx
+// along x and cell_edge/K along y and z; the stencil reaches Kx cells along x and K
+// along y and z), particles per workgroup and LDS queue entries per particle (depth 1:
+// no queue, the pair arithmetic runs in the candidate loop).
 struct Shape {
-    int refine, block, depth;
+    int refine, refine_x, block, depth;
     bool queued() const { return depth > 1; }
     size_t lds_bytes() const { return queued() ? (size_t)depth * (size_t)block * sizeof(uint32_t) : 0; }
 };
@@ -60,16 +61,16 @@ __device__ inline float axis_gap(float v, int32_t cc, int32_t home, float lo, fl
 }
 
 // Particles in cell c of the cell_edge grid cg, from the offsets foff of the refined
-// grid fg (edge cg.cw / K).  For K = 2 or 4, x / (cw/K) is exactly K (x / cw) in binary
-// floating point and floor(floor(K q) / K) = floor(q), so coarse cell X holds fine
-// cells KX .. KX+K-1 (and the extra last fine cells, which the coarse clamp also maps
-// to the last X).
-__device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const GridGeom& fg, const GridGeom& cg, int K,
-                                        uint32_t c) {
+// grid fg (edge cg.cw / Kx along x, cg.cw / K along y and z).  For a power of two K,
+// x / (cw/K) is exactly K (x / cw) in binary floating point and floor(floor(K q) / K) =
+// floor(q), so coarse cell X holds fine cells KX .. KX+K-1 (and the extra last fine
+// cells, which the coarse clamp also maps to the last X).
+__device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const GridGeom& fg, const GridGeom& cg, int Kx,
+                                        int K, uint32_t c) {
     const int32_t X = (int32_t)(c % (uint32_t)cg.g[0]);
     const uint32_t rem = c / (uint32_t)cg.g[0];
     const int32_t Y = (int32_t)(rem % (uint32_t)cg.g[1]), Z = (int32_t)(rem / (uint32_t)cg.g[1]);
-    const int32_t fx0 = K * X, fx1 = X == cg.g[0] - 1 ? fg.g[0] - 1 : K * X + K - 1;
+    const int32_t fx0 = Kx * X, fx1 = X == cg.g[0] - 1 ? fg.g[0] - 1 : Kx * X + Kx - 1;
     const int32_t fy0 = K * Y, fy1 = Y == cg.g[1] - 1 ? fg.g[1] - 1 : K * Y + K - 1;
     const int32_t fz0 = K * Z, fz1 = Z == cg.g[2] - 1 ? fg.g[2] - 1 : K * Z + K - 1;
     uint32_t s = 0;
@@ -83,8 +84,8 @@ __device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const
 
 // Accepted partners of the particle in sorted slot k (position pi, cell `key` of grid
 // g, whose cell 0 is global cell `cbase`), summed into acc; returns their number.
-// The stencil is (2R+1)^3 cells of g, visited as (2R+1)^2 x-rows in z -> y order,
-// rows and x-cells beyond the cutoff trimmed.
+// The stencil is (2Rx+1) x (2R+1)^2 cells of g, visited as (2R+1)^2 x-rows in z -> y
+// order, rows and x-cells beyond the cutoff trimmed.
 //
 // The candidate loop only evaluates the predicate.  An accepted partner's slot goes
 // to the lane's queue in LDS (entry q of the lane at queue[q * stride]: lanes of a
@@ -101,7 +102,7 @@ __device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const
 template <int PRED, int PREC, bool VIR, bool QUEUE>
 __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
                                          bool active, const float4& pi, const GridGeom& g, uint32_t cbase,
-                                         uint32_t key, int R, const Trim& tr, const PairConst& c,
+                                         uint32_t key, int Rx, int R, const Trim& tr, const PairConst& c,
                                          uint32_t* __restrict__ queue, uint32_t stride, unsigned depth,
                                          PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR>& acc) {
     unsigned pc = 0;          // queued partners of this lane
@@ -147,17 +148,17 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
     if (active) cell_coords(g, key, cx, cy, cz);
     const bool trim = active && tr.on && pi.x >= g.lo[0] && pi.y >= g.lo[1] && pi.z >= g.lo[2] &&
                       pi.x <= tr.hi[0] && pi.y <= tr.hi[1] && pi.z <= tr.hi[2];
-    const int32_t x0 = cx - R > 0 ? cx - R : 0, x1 = cx + R < g.g[0] - 1 ? cx + R : g.g[0] - 1;
+    const int32_t x0 = cx - Rx > 0 ? cx - Rx : 0, x1 = cx + Rx < g.g[0] - 1 ? cx + Rx : g.g[0] - 1;
     // an inactive lane has an empty stencil (z0 > z1) but still takes part in the flushes
     const int32_t z0 = active ? (cz - R > 0 ? cz - R : 0) : 1;
     const int32_t z1 = active ? (cz + R < g.g[2] - 1 ? cz + R : g.g[2] - 1) : 0;
     const int32_t y0 = cy - R > 0 ? cy - R : 0, y1 = cy + R < g.g[1] - 1 ? cy + R : g.g[1] - 1;
     for (int32_t zz = z0; zz <= z1; ++zz) {
-        const float gz = trim ? axis_gap(pi.z, zz, cz, g.lo[2], g.cw, tr.margin) : 0.f;
+        const float gz = trim ? axis_gap(pi.z, zz, cz, g.lo[2], g.cw[2], tr.margin) : 0.f;
         for (int32_t yy = y0; yy <= y1; ++yy) {
             int32_t xa = x0, xb = x1;
             if (trim) {
-                const float gy = axis_gap(pi.y, yy, cy, g.lo[1], g.cw, tr.margin);
+                const float gy = axis_gap(pi.y, yy, cy, g.lo[1], g.cw[1], tr.margin);
                 const float w2 = tr.r2 - gz * gz - gy * gy;
                 if (w2 < 0.f) continue;
                 const float w = __builtin_sqrtf(w2) + tr.margin;
@@ -184,7 +185,7 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
 }
 
 // Host side (force.hip).
-int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int K, GridGeom* f);
+int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int Kx, int K, GridGeom* f);
 Trim make_trim(const lj_params* p, const GridGeom& g);
 // Shape of a call over n particles in ncells cells of the cell_edge grid: explicit
 // fields of p (validated by make_geom) win; zero fields take the defaults.

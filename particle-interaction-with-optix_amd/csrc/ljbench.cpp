@@ -345,7 +345,7 @@ int run_lj(int argc, char** argv) {
     long long n = 100000;
     double rc = 0.04;
     std::string gen = "uniform", prec = "mixed";
-    int loops = 5, refine = 0, block = 0, depth = 0;
+    int loops = 5, refine = 0, refine_x = 0, block = 0, depth = 0;
     bool lbvh = false, tune = false;
     uint64_t seed = 1234;
     for (int i = 2; i < argc; ++i) {
@@ -362,6 +362,7 @@ int run_lj(int argc, char** argv) {
         else if (a == "--gen") gen = next();
         else if (a == "--loops") loops = std::atoi(next());
         else if (a == "--refine") refine = std::atoi(next());
+        else if (a == "--refine-x") refine_x = std::atoi(next());
         else if (a == "--block") block = std::atoi(next());
         else if (a == "--depth") depth = std::atoi(next());
         else if (a == "--autotune") tune = true;
@@ -387,7 +388,8 @@ int run_lj(int argc, char** argv) {
         CHECK_LJ(ctx, lj_generate(LJ_GEN_UNIFORM, seed, 0, n, nullptr, x.data(), y.data(), z.data()));
     }
     lj_params p = base_params((float)rc, (float)(rc / 2.5), 1.0f, lo, hi, LJ_PRED_AABB);
-    p.refine = refine;
+    p.refine = (int16_t)refine;
+    p.refine_x = (int16_t)refine_x;
     p.block = (int16_t)block;
     p.depth = (int16_t)depth;
     p.precision = prec == "fp32" ? LJ_PREC_FP32 : prec == "fp32_comp" ? LJ_PREC_FP32_COMP : LJ_PREC_MIXED;
@@ -397,8 +399,8 @@ int run_lj(int argc, char** argv) {
     if (tune && !lbvh) {
         lj_tuning t;
         CHECK_LJ(ctx, lj_autotune(ctx, dev.x, dev.y, dev.z, n, &p, 3, &t, nullptr));
-        std::printf("autotune: refine %d, block %d, depth %d (%d shapes; %.4f ms vs default %.4f ms)\n", t.refine,
-                    t.block, t.depth, t.n_tried, t.ms_best, t.ms_default);
+        std::printf("autotune: refine %d, refine_x %d, block %d, depth %d (%d shapes; %.4f ms vs default %.4f ms)\n",
+                    t.refine, t.refine_x, t.block, t.depth, t.n_tried, t.ms_best, t.ms_default);
     }
     (void)timed(ctx, lbvh, dev, n, p, nullptr);  // warm-up (grows the workspace)
     double ti = 0, tc = 0;
@@ -423,6 +425,7 @@ void usage(const char* argv0) {
                  "       %s aabb [-rb|--runbench] [-gs|--gensurface] [-us|--usedsort] [-ns|--nbspheres N]\n"
                  "            [-bd|--boxdivisor D] [--max-boxdiv M]\n"
                  "       %s lj [--n N] [--rc RC] [--gen uniform|surface|clusters] [--loops L] [--refine K]\n"
+                 "            [--refine-x KX]\n"
                  "            [--block B] [--depth D] [--autotune] [--precision mixed|fp32|fp32_comp] [--lbvh]\n"
                  "            [--seed S]\n",
                  argv0, argv0, argv0);

@@ -1,13 +1,15 @@
 // tune.hip -- the autotuner: measured choice of the force-kernel shape.
 //
-// The shape of a force launch is the force-grid refinement K (cells of edge
-// cell_edge / K, stencil radius K), the particles per workgroup and the depth of the
-// per-lane LDS queues.  None of them changes which pairs are found; they trade
+// The shape of a force launch is the force-grid refinement (cells of edge
+// cell_edge / K along y and z and cell_edge / Kx along x), the particles per workgroup
+// and the depth of the per-lane LDS queues.  None of them changes which pairs are found; they trade
 // candidate tests (fewer with finer cells) against cell-offset reads and loop
 // overhead (more with finer cells), and occupancy against queue fill.  The best
 // choice depends on the density and distribution of the particles, so it is
-// measured: every valid shape runs `reps` times on the caller's positions after one
-// untimed call, timed with hipEvents on the caller's stream, and the fastest wins.
+// measured, in two stages: every valid grid (K, Kx) with the default workgroup and no
+// queue, then every workgroup size and queue depth on the fastest grid.  Each shape
+// runs `reps` times on the caller's positions after one untimed call, timed with
+// hipEvents on the caller's stream, and the fastest wins.
 #include <vector>
 
 #include "lj_stencil.h"
@@ -16,6 +18,7 @@ namespace lj {
 namespace {
 
 constexpr int kRefines[] = {1, 2, 4};
+constexpr int kRefinesX[] = {1, 2, 4, 8, 16};
 constexpr int kBlocks[] = {64, 128, 256};
 constexpr int kDepths[] = {1, 8, 16, 32};  // 1: no queue
 
@@ -57,28 +60,39 @@ int search(lj_context* ctx, hipStream_t s, int reps, bool cudalj, Run run, lj_tu
     LJ_HIP(ctx, hipEventCreate(&ev.b));
     ProfilerOff off(ctx);
     float ms = 0.f;
-    LJ_TRY(time_calls(ctx, ev, s, reps, [&] { return run(0, 0, 0); }, &ms));
+    LJ_TRY(time_calls(ctx, ev, s, reps, [&] { return run(0, 0, 0, 0); }, &ms));
     *best = lj_tuning{};
     best->ms_default = ms;
     best->ms_best = -1.f;
-    for (int K : kRefines) {
-        if (cudalj && K != 1) continue;  // the reference stencil is the 27 cells
-        for (int b : kBlocks)
-            for (int d : kDepths) {
-                const int r = time_calls(ctx, ev, s, reps, [&] { return run(K, b, d); }, &ms);
-                if (r == LJ_ERR_OVERFLOW) goto next_refine;  // finer grid too large for this workload
-                LJ_TRY(r);
-                ++best->n_tried;
-                if (best->ms_best < 0.f || ms < best->ms_best) {
-                    best->ms_best = ms;
-                    best->refine = K;
-                    best->block = b;
-                    best->depth = d;
-                }
-            }
-    next_refine:;
-    }
+    auto consider = [&](int K, int Kx, int b, int d) {
+        const int r = time_calls(ctx, ev, s, reps, [&] { return run(K, Kx, b, d); }, &ms);
+        if (r != LJ_OK) return r;
+        ++best->n_tried;
+        if (best->ms_best < 0.f || ms < best->ms_best) {
+            best->ms_best = ms;
+            best->refine = K;
+            best->refine_x = Kx;
+            best->block = b;
+            best->depth = d;
+        }
+        return LJ_OK;
+    };
+    // stage 1: grids (the CUDALJ mode keeps the reference's 27 cells)
+    for (int K : kRefines)
+        for (int Kx : kRefinesX) {
+            if (Kx < K || (cudalj && (K != 1 || Kx != 1))) continue;
+            const int r = consider(K, Kx, 256, 1);
+            if (r == LJ_ERR_OVERFLOW) continue;  // grid too fine for this workload
+            LJ_TRY(r);
+        }
     if (best->n_tried == 0) return fail(ctx, LJ_ERR_INVALID_ARG, "no valid kernel shape for this workload");
+    // stage 2: workgroup size and queue depth on the fastest grid
+    const int K = best->refine, Kx = best->refine_x;
+    for (int b : kBlocks)
+        for (int d : kDepths) {
+            if (b == 256 && d == 1) continue;  // timed in stage 1
+            LJ_TRY(consider(K, Kx, b, d));
+        }
     return LJ_OK;
 }
 
@@ -105,8 +119,9 @@ int lj_autotune(lj_context* ctx, const float* x, const float* y, const float* z,
     float* f = ctx->tune_out.as<float>();
     lj_result* res = reinterpret_cast<lj_result*>(ctx->tune_out.as<uint8_t>() + ((3 * nn * sizeof(float) + 255) & ~size_t(255)));
     lj_params q = *p;
-    auto run = [&](int K, int b, int d) {
-        q.refine = K;
+    auto run = [&](int K, int Kx, int b, int d) {
+        q.refine = (int16_t)K;
+        q.refine_x = (int16_t)Kx;
         q.block = (int16_t)b;
         q.depth = (int16_t)d;
         GridGeom gq;
@@ -116,7 +131,8 @@ int lj_autotune(lj_context* ctx, const float* x, const float* y, const float* z,
     };
     lj_tuning best;
     LJ_TRY(search(ctx, s, reps, p->predicate == LJ_PRED_CUDALJ, run, &best));
-    p->refine = best.refine;
+    p->refine = (int16_t)best.refine;
+    p->refine_x = (int16_t)best.refine_x;
     p->block = (int16_t)best.block;
     p->depth = (int16_t)best.depth;
     if (out) *out = best;
@@ -142,9 +158,10 @@ int lj_autotune_batched(lj_context* ctx, lj_system* sys, int32_t nsys, const flo
     float* f = ctx->tune_out.as<float>();
     lj_result* res = reinterpret_cast<lj_result*>(ctx->tune_out.as<uint8_t>() + fbytes);
     std::vector<lj_system> q(sys, sys + nsys);
-    auto run = [&](int K, int b, int d) {
+    auto run = [&](int K, int Kx, int b, int d) {
         for (auto& t : q) {
-            t.params.refine = K;
+            t.params.refine = (int16_t)K;
+            t.params.refine_x = (int16_t)Kx;
             t.params.block = (int16_t)b;
             t.params.depth = (int16_t)d;
         }
@@ -153,7 +170,8 @@ int lj_autotune_batched(lj_context* ctx, lj_system* sys, int32_t nsys, const flo
     lj_tuning best;
     LJ_TRY(search(ctx, s, reps, sys[0].params.predicate == LJ_PRED_CUDALJ, run, &best));
     for (int32_t i = 0; i < nsys; ++i) {
-        sys[i].params.refine = best.refine;
+        sys[i].params.refine = (int16_t)best.refine;
+        sys[i].params.refine_x = (int16_t)best.refine_x;
         sys[i].params.block = (int16_t)best.block;
         sys[i].params.depth = (int16_t)best.depth;
     }

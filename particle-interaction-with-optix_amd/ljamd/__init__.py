@@ -73,7 +73,8 @@ class LJParams(ctypes.Structure):
         ("predicate", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("flags", ctypes.c_uint32),
-        ("refine", ctypes.c_int32),
+        ("refine", ctypes.c_int16),
+        ("refine_x", ctypes.c_int16),
         ("block", ctypes.c_int16),
         ("depth", ctypes.c_int16),
     ]
@@ -95,15 +96,17 @@ class LJSystem(ctypes.Structure):
 class LJTuning(ctypes.Structure):
     _fields_ = [
         ("refine", ctypes.c_int32),
+        ("refine_x", ctypes.c_int32),
         ("block", ctypes.c_int32),
         ("depth", ctypes.c_int32),
         ("n_tried", ctypes.c_int32),
         ("ms_best", ctypes.c_float),
         ("ms_default", ctypes.c_float),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
 RESULT_DTYPE = np.dtype(
@@ -181,9 +184,10 @@ def make_params(
     refine: int = 0,
     block: int = 0,
     depth: int = 0,
+    refine_x: int = 0,
 ) -> LJParams:
     """Run parameters; sigma defaults to rc / 2.5 (the workloads' choice).  refine,
-    block and depth are the kernel shape (0 = default; see Engine.autotune)."""
+    refine_x, block and depth are the kernel shape (0 = default; see Engine.autotune)."""
     p = LJParams()
     p.rc = rc
     p.sigma = rc / 2.5 if sigma is None else sigma
@@ -197,6 +201,7 @@ def make_params(
     p.precision = PREC[precision] if isinstance(precision, str) else int(precision)
     p.flags = flags
     p.refine = refine
+    p.refine_x = refine_x
     p.block = block
     p.depth = depth
     return p
@@ -215,6 +220,7 @@ def params_to_dict(p: LJParams) -> dict:
         "precision": p.precision,
         "flags": p.flags,
         "refine": p.refine,
+        "refine_x": p.refine_x,
         "block": p.block,
         "depth": p.depth,
     }
@@ -394,7 +400,7 @@ class Engine:
 
     def autotune(self, x, y, z, p: LJParams, reps: int = 3) -> dict:
         """Times every valid kernel shape on these positions and writes the fastest
-        into p (refine, block, depth); returns the lj_tuning record as a dict."""
+        into p (refine, refine_x, block, depth); returns the lj_tuning record as a dict."""
         t = LJTuning()
         self._check(
             self.lib.lj_autotune(self.ctx, _ptr(x), _ptr(y), _ptr(z), x.numel(), ctypes.byref(p), reps, ctypes.byref(t), self._stream())
@@ -410,6 +416,7 @@ class Engine:
         )
         for q, sy in enumerate(systems):
             sy.params.refine = arr[q].params.refine
+            sy.params.refine_x = arr[q].params.refine_x
             sy.params.block = arr[q].params.block
             sy.params.depth = arr[q].params.depth
         return t.as_dict()

@@ -33,8 +33,8 @@ def test_struct_layouts_match_header_sizes():
     assert ctypes.sizeof(ljamd.LJParams) == 64
     assert ctypes.sizeof(ljamd.LJResult) == 32
     assert ctypes.sizeof(ljamd.LJSystem) == 16 + 64
-    assert ctypes.sizeof(ljamd.LJTuning) == 24
-    assert ljamd.LJParams.refine.offset == 56
+    assert ctypes.sizeof(ljamd.LJTuning) == 32
+    assert ljamd.LJParams.refine.offset == 56 and ljamd.LJParams.refine_x.offset == 58
     assert ljamd.LJParams.block.offset == 60 and ljamd.LJParams.depth.offset == 62
     # the oracle's params are the leading fields of the engine's (same offsets)
     for name, _ in po.OrcParams._fields_:
@@ -62,6 +62,8 @@ def test_grid_dims_rejects_bad_params():
         ljamd.make_params(0.1, refine=3),
         ljamd.make_params(0.1, block=96),
         ljamd.make_params(0.1, depth=12),
+        ljamd.make_params(0.1, refine_x=3),
+        ljamd.make_params(0.1, refine=4, refine_x=2),
     ):
         with pytest.raises(ljamd.LJError):
             ljamd.grid_dims(p)

@@ -103,14 +103,15 @@ def test_forces_mixed_vs_fp64(engine, name):
         assert res["virial"] == pytest.approx(ref["virial"], rel=1e-9, abs=1e-12)
 
 
-@pytest.mark.parametrize("refine", [1, 2, 4])
+@pytest.mark.parametrize("refine,refine_x", [(1, 1), (2, 2), (4, 4), (1, 4), (1, 8), (1, 16), (2, 8), (4, 16)])
 @pytest.mark.parametrize("name", ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "edges", "one_cell_crowd", "tiny_7"])
-def test_refined_grids_same_pairs_and_forces(engine, name, refine):
+def test_refined_grids_same_pairs_and_forces(engine, name, refine, refine_x):
     """The 27-cell grid and the half/quarter-edge grids with stencil trimming visit
     every accepted pair: counts bit-exact, forces within the bar."""
     (x, y, z), p = _case(name)
     p.refine = refine
-    print(f"[refine] {name} k={refine}", flush=True)
+    p.refine_x = refine_x
+    print(f"[refine] {name} k={refine} kx={refine_x}", flush=True)
     ref = po.forces_f64(po.from_lj(p), x, y, z)
     f, e, res = _forces(engine, x, y, z, p)
     assert int(res["n_pairs"]) == ref["n_pairs"]
@@ -153,8 +154,9 @@ def test_autotune_picks_a_valid_shape(engine):
     print("[autotune] c1", flush=True)
     t = engine.autotune(dx, dy, dz, p, reps=2)
     assert t["refine"] in (1, 2, 4) and t["block"] in (64, 128, 256) and t["depth"] in (1, 8, 16, 32)
-    assert (p.refine, p.block, p.depth) == (t["refine"], t["block"], t["depth"])
-    assert t["n_tried"] == 36
+    assert t["refine_x"] in (1, 2, 4, 8, 16) and t["refine_x"] >= t["refine"]
+    assert (p.refine, p.refine_x, p.block, p.depth) == (t["refine"], t["refine_x"], t["block"], t["depth"])
+    assert t["n_tried"] == 12 + 11
     assert 0 < t["ms_best"] and 0 < t["ms_default"]
     ref = po.forces_f64(po.from_lj(p), x, y, z)
     f, e, res = _forces(engine, x, y, z, p)
@@ -169,8 +171,8 @@ def test_autotune_cudalj_keeps_the_27_cells(engine):
     q = ljamd.make_params(float(p.rc), sigma=1.0, softening2=1e-5, predicate="cudalj")
     dx, dy, dz = engine.to_device(x[:20000], y[:20000], z[:20000])
     t = engine.autotune(dx, dy, dz, q, reps=1)
-    assert t["refine"] == 1 and q.refine == 1
-    assert t["n_tried"] == 12
+    assert t["refine"] == 1 and q.refine == 1 and t["refine_x"] == 1
+    assert t["n_tried"] == 1 + 11
 
 
 def test_trimming_at_cell_faces(engine):
@@ -183,8 +185,8 @@ def test_trimming_at_cell_faces(engine):
         pts.append(base + np.array(dxyz, np.float32))
     pts = np.array(pts, np.float32)
     x, y, z = (np.ascontiguousarray(pts[:, d]) for d in range(3))
-    for refine in (1, 2, 4):
-        p = ljamd.make_params(float(rc), refine=refine)
+    for refine, refine_x in ((1, 1), (2, 2), (4, 4), (1, 8), (2, 16)):
+        p = ljamd.make_params(float(rc), refine=refine, refine_x=refine_x)
         rows, cols = po.pairs_brute(po.from_lj(p), x, y, z)
         f, e, res = _forces(engine, x, y, z, p)
         assert int(res["n_pairs"]) == int(rows[-1]), refine

@@ -88,14 +88,15 @@ def _small_batch(nsys=48):
     return workloads.batch_c4(nsys=nsys, nmin=64, nmax=4096, seed=77)
 
 
-@pytest.mark.parametrize("refine", [1, 2, 4])
-def test_batched_equals_single_calls_and_oracle(engine, refine):
+@pytest.mark.parametrize("refine,refine_x", [(1, 1), (2, 2), (4, 4), (1, 8)])
+def test_batched_equals_single_calls_and_oracle(engine, refine, refine_x):
     """One launch over all systems gives each system the bits of its own call with
     the same kernel shape, and the oracle's counts, totals and forces."""
     systems, X, Y, Z = _small_batch()
     for sy in systems:
         sy.params.refine = refine
-    print(f"[batched] refine={refine}", flush=True)
+        sy.params.refine_x = refine_x
+    print(f"[batched] refine={refine} refine_x={refine_x}", flush=True)
     dx, dy, dz = engine.to_device(X, Y, Z)
     out = engine.compute_batched(systems, dx, dy, dz)
     engine.torch.cuda.synchronize()
@@ -130,8 +131,8 @@ def test_batched_autotune(engine):
     print("[batched autotune]", flush=True)
     t = engine.autotune_batched(systems, dx, dy, dz, reps=2)
     assert t["refine"] in (1, 2, 4) and t["block"] in (64, 128, 256) and t["depth"] in (1, 8, 16, 32)
-    assert all((sy.params.refine, sy.params.block, sy.params.depth) == (t["refine"], t["block"], t["depth"])
-               for sy in systems)
+    assert all((sy.params.refine, sy.params.refine_x, sy.params.block, sy.params.depth) ==
+               (t["refine"], t["refine_x"], t["block"], t["depth"]) for sy in systems)
     out = engine.compute_batched(systems, dx, dy, dz)
     engine.torch.cuda.synchronize()
     res1 = ljamd.read_results(out["result"], len(systems))
