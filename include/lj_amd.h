@@ -182,6 +182,23 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
                   const lj_params* p, unsigned long long* row_offsets, uint32_t* cols,
                   unsigned long long capacity, unsigned long long* needed, void* stream);
 
+/* Device: forces and energies over a stored neighbour list (CSR by original index, as
+ * lj_pairs / lj_lbvh_pairs return it; entries >= n are ignored), with p's predicate
+ * applied to every listed pair.  Built at rc + skin, the list holds every pair within
+ * rc as long as no particle has moved more than skin / 2 since the build
+ * (lj_max_displacement), and the call then finds exactly lj_compute's pairs.  Outputs
+ * as lj_compute; result->n_candidates = listed pairs.  No reference counterpart: the
+ * reference regenerates its particles every loop (Cuda-version/cudaLJ.cu:602-611). */
+int lj_compute_list(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
+                    const lj_params* p, const unsigned long long* row_offsets, const uint32_t* cols,
+                    float* fx, float* fy, float* fz, float* e, lj_result* result, void* stream);
+
+/* Device: the largest squared displacement |p - p0|^2 over n particles, into *max2
+ * (one device float; NaN displacements read as infinite). */
+int lj_max_displacement(lj_context* ctx, const float* x, const float* y, const float* z,
+                        const float* x0, const float* y0, const float* z0, int64_t n, float* max2,
+                        void* stream);
+
 /* One system of a batch: particles [first, first+n) of the concatenated arrays. */
 typedef struct lj_system {
     int64_t first;

@@ -57,6 +57,8 @@ EXPORTS = (
     "lj_profile_read",
     "lj_autotune",
     "lj_autotune_batched",
+    "lj_compute_list",
+    "lj_max_displacement",
 )
 PHASES = ("bin", "force", "other")
 
@@ -159,6 +161,8 @@ def load(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
         "lj_profile_read": ([vp, P(ctypes.c_double), P(ctypes.c_ulonglong)], ctypes.c_int),
         "lj_autotune": ([vp, vp, vp, vp, i64, P(LJParams), ctypes.c_int32, P(LJTuning), vp], ctypes.c_int),
         "lj_autotune_batched": ([vp, P(LJSystem), ctypes.c_int32, vp, vp, vp, ctypes.c_int32, P(LJTuning), vp], ctypes.c_int),
+        "lj_compute_list": ([vp, vp, vp, vp, i64, P(LJParams), vp, vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+        "lj_max_displacement": ([vp, vp, vp, vp, vp, vp, vp, i64, vp, vp], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         if hasattr(lib, name):  # tests check that every declared symbol is exported
@@ -395,6 +399,34 @@ class Engine:
         self._check(
             self.lib.lj_compute_batched(self.ctx, arr, len(systems), _ptr(x), _ptr(y), _ptr(z), _ptr(out["fx"]), _ptr(out["fy"]),
                                         _ptr(out["fz"]), _ptr(out["e"]), _ptr(out["result"]), self._stream())
+        )
+        return out
+
+    def compute_list(self, x, y, z, p: LJParams, rows, cols, energies: bool = True, out=None):
+        """Forces over a stored CSR neighbour list (see pairs()); same outputs as compute()."""
+        n = x.numel()
+        if out is None:
+            out = {
+                "fx": self.empty_f32(n),
+                "fy": self.empty_f32(n),
+                "fz": self.empty_f32(n),
+                "e": self.empty_f32(n) if energies else None,
+                "result": self.empty_result(),
+            }
+        self._check(
+            self.lib.lj_compute_list(self.ctx, _ptr(x), _ptr(y), _ptr(z), n, ctypes.byref(p), _ptr(rows), _ptr(cols),
+                                     _ptr(out["fx"]), _ptr(out["fy"]), _ptr(out["fz"]), _ptr(out["e"]),
+                                     _ptr(out["result"]), self._stream())
+        )
+        return out
+
+    def max_displacement2(self, x, y, z, x0, y0, z0, out=None):
+        """Device float tensor holding max |p - p0|^2."""
+        if out is None:
+            out = self.torch.empty(1, dtype=self.torch.float32, device=x.device)
+        self._check(
+            self.lib.lj_max_displacement(self.ctx, _ptr(x), _ptr(y), _ptr(z), _ptr(x0), _ptr(y0), _ptr(z0), x.numel(),
+                                         _ptr(out), self._stream())
         )
         return out
 
