@@ -14,8 +14,8 @@
 //     Octrees, and k-d Trees", HPG 2012 (one thread per internal node);
 //  3. node bounds without synchronisation: an internal node covers a contiguous range
 //     of leaves (Morton order), so its bounds are the min/max over that range, read
-//     from three levels of aligned chunk bounds (256, 65536 and 2^24 leaves) plus
-//     the leaves at the range's ends.  (Merging child boxes bottom-up would need an
+//     from six levels of aligned chunk bounds (16, 256, ... 2^24 leaves) plus the
+//     leaves at the range's ends.  (Merging child boxes bottom-up would need an
 //     acquire/release handshake per node across workgroups; at device scope on a
 //     multi-XCD part each one writes back and invalidates the L2.)
 //  4. one thread per particle (in Morton order) walks the tree without a stack: every
@@ -119,8 +119,9 @@ __global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, uint32_t 
     if (i < n) p[i] = v;
 }
 
-constexpr int kChunkBits = 8;  // a chunk of level l + 1 holds 256 chunks of level l
-constexpr int kLevels = 3;     // chunks of 2^8, 2^16 and 2^24 leaves
+constexpr int kChunkBits = 4;  // a chunk of level l + 1 holds 16 chunks of level l
+constexpr int kChunk = 1 << kChunkBits;
+constexpr int kLevels = 6;     // chunks of 16, 256, 4096, 65536, 2^20 and 2^24 leaves
 
 struct Bounds {
     float lo[3], hi[3];
@@ -138,11 +139,12 @@ struct Bounds {
     }
 };
 
-// Bounds of aligned chunks: chunk c of the output covers inputs [256 c, 256 c + 256)
-// (leaf positions when `leaves`, else (lo, hi) pairs of the level below).
+// Bounds of aligned chunks: chunk c of the output covers inputs [16 c, 16 c + 16)
+// (leaf positions when `leaves`, else (lo, hi) pairs of the level below); 16 lanes per
+// chunk.
 __global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__ in, bool leaves, uint32_t count,
-                                                      float4* __restrict__ out) {
-    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+                                                      uint32_t nchunks, float4* __restrict__ out) {
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;  // input element
     Bounds b;
     b.empty();
     if (e < count) {
@@ -150,39 +152,27 @@ __global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__
         else b.add(in[2 * e], in[2 * e + 1]);
     }
 #pragma unroll
-    for (int d = kWaveSize / 2; d > 0; d >>= 1)
+    for (int d = kChunk / 2; d > 0; d >>= 1)
         for (int a = 0; a < 3; ++a) {
-            b.lo[a] = fminf(b.lo[a], __shfl_xor(b.lo[a], d, kWaveSize));
-            b.hi[a] = fmaxf(b.hi[a], __shfl_xor(b.hi[a], d, kWaveSize));
+            b.lo[a] = fminf(b.lo[a], __shfl_xor(b.lo[a], d, kChunk));
+            b.hi[a] = fmaxf(b.hi[a], __shfl_xor(b.hi[a], d, kChunk));
         }
-    __shared__ float sh[256 / kWaveSize][6];
-    const unsigned w = threadIdx.x / kWaveSize;
-    if (__lane_id() == 0)
-        for (int a = 0; a < 3; ++a) {
-            sh[w][a] = b.lo[a];
-            sh[w][3 + a] = b.hi[a];
-        }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (unsigned q = 1; q < 256 / kWaveSize; ++q)
-            for (int a = 0; a < 3; ++a) {
-                b.lo[a] = fminf(b.lo[a], sh[q][a]);
-                b.hi[a] = fmaxf(b.hi[a], sh[q][3 + a]);
-            }
-        out[2 * blockIdx.x] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.f);
-        out[2 * blockIdx.x + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.f);
+    const uint32_t c = e / kChunk;
+    if (e % kChunk == 0 && c < nchunks) {
+        out[2 * c] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.f);
+        out[2 * c + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.f);
     }
 }
 
 // Bounds of internal node i: min/max of the leaf positions in its range, from the
-// largest aligned chunks inside the range and single leaves at its ends.  chunk[l] is
-// level l + 1 (chunks of 2^(8 (l + 1)) leaves); counts are the full chunks per level.
-// The .w lanes carry the walk: lo.w = the left child, hi.w = the rope (references as
-// int bits).
+// largest aligned chunks inside the range and single leaves at its ends (at most 15
+// elements per level on each side).  Level l (0 .. kLevels-1, chunks of
+// 16^(l + 1) leaves) starts at chunks + 2 * l * stride.  The .w lanes carry the walk:
+// lo.w = the left child, hi.w = the rope (references as int bits).
 __global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ leaf, int n,
                                                      const int2* __restrict__ range, const int2* __restrict__ child,
-                                                     const int* __restrict__ rope_by_end, const float4* __restrict__ c1,
-                                                     const float4* __restrict__ c2, const float4* __restrict__ c3,
+                                                     const int* __restrict__ rope_by_end,
+                                                     const float4* __restrict__ chunks, uint32_t stride,
                                                      float4* __restrict__ box) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -191,14 +181,14 @@ __global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ 
     const uint32_t b = (uint32_t)r.y + 1u;  // exclusive, <= n
     Bounds bb;
     bb.empty();
-    // element e of level l: a leaf (l = -1) or chunk e of level l (2^(8 (l + 1)) leaves)
+    // element e of level l: a leaf (l = -1) or chunk e of level l
     auto add = [&](int l, uint32_t e) {
         if (l < 0) {
             const float4 p = leaf[e];
             bb.add(p, p);
         } else {
-            const float4* c = l == 0 ? c1 : (l == 1 ? c2 : c3);
-            bb.add(c[2 * e], c[2 * e + 1]);
+            const float4* c = chunks + 2 * ((size_t)l * stride + e);
+            bb.add(c[0], c[1]);
         }
     };
     // Climb: while a whole chunk of the next level lies inside [a, b), consume the
@@ -372,8 +362,9 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     LJ_HIP(ctx, L.rope.grow(nn * sizeof(int)));
     LJ_HIP(ctx, L.flags.grow(64));
     LJ_HIP(ctx, L.node_box.grow(nn * 2 * sizeof(float4)));
-    // chunk bounds of the three levels: ceil(n/256) + ceil(n/2^16) + ceil(n/2^24) pairs
-    LJ_HIP(ctx, L.chunks.grow((nn / 256 + nn / 65536 + nn / (1u << 24) + 3) * 2 * sizeof(float4)));
+    // chunk bounds: kLevels levels of ceil(n / 16) pairs each (the higher levels use
+    // only their first entries)
+    LJ_HIP(ctx, L.chunks.grow((size_t)kLevels * div_up((int64_t)nn, kChunk) * 2 * sizeof(float4)));
     MortonKeyOp op;
     for (int d = 0; d < 3; ++d) {
         op.lo[d] = p->box_lo[d];
@@ -390,27 +381,26 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
         hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, s, code, (int)n, L.node_child.as<int2>(),
                            range, rope);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, overflow_flag(ctx, n), 1u, 0);
-    // chunk bounds, level by level (level l + 1 from level l; level 1 from the leaves)
+    // chunk bounds, level by level (level 0 from the leaves, level l + 1 from level l);
+    // level l holds ceil(n / 16^(l + 1)) pairs at chunks + 2 * l * stride
     const float4* leaves = B.pos.as<float4>();
-    float4* lvl[3];
+    const uint32_t stride = (uint32_t)div_up(n, kChunk);
     {
-        uint64_t cnt = (uint64_t)n, off = 0;
+        uint64_t cnt = (uint64_t)n;
         const float4* in = leaves;
-        for (int l = 0; l < 3; ++l) {
-            const uint64_t chunks = div_up(cnt, 256);
-            lvl[l] = L.chunks.as<float4>() + 2 * off;
-            hipLaunchKernelGGL(k_chunk_bounds, dim3((unsigned)chunks), dim3(256), 0, s, in, l == 0, (uint32_t)cnt,
-                               lvl[l]);
-            in = lvl[l];
-            cnt = chunks;
-            off += chunks;
+        for (int l = 0; l < kLevels; ++l) {
+            const uint64_t nch = div_up(cnt, kChunk);
+            float4* out = L.chunks.as<float4>() + 2 * (size_t)l * stride;
+            hipLaunchKernelGGL(k_chunk_bounds, dim3((unsigned)div_up(cnt, 256)), dim3(256), 0, s, in, l == 0,
+                               (uint32_t)cnt, (uint32_t)nch, out);
+            in = out;
+            cnt = nch;
         }
     }
     if (n > 1)
         hipLaunchKernelGGL(k_node_bounds, dim3(div_up(n - 1, 256)), dim3(256), 0, s, leaves, (int)n,
                            (const int2*)range, (const int2*)L.node_child.as<int2>(), (const int*)rope,
-                           (const float4*)lvl[0], (const float4*)lvl[1], (const float4*)lvl[2],
-                           L.node_box.as<float4>());
+                           (const float4*)L.chunks.as<float4>(), stride, L.node_box.as<float4>());
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }
