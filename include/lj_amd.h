@@ -93,8 +93,8 @@ typedef struct lj_params {
      * forces and energies agree to the rounding of the final FP32 outputs. */
     int16_t refine;    /* force-grid cells per cell_edge along y and z: 0, 1, 2 or 4;
                           the candidate count keeps the cell_edge grid of the reference */
-    int16_t refine_x;  /* force-grid cells per cell_edge along x: 0 (= refine), 1, 2, 4, 8
-                          or 16, not below refine */
+    int16_t refine_x;  /* force-grid cells per cell_edge along x: 1, 2, 4, 8 or 16, not
+                          below refine; 0 = refine when refine is set, else the default */
     int16_t block;     /* particles per workgroup: 0, 64, 128 or 256 */
     int16_t depth;     /* LDS queue entries per particle: 0, 8, 16 or 32; 1 = no queue */
 } lj_params;

@@ -241,23 +241,21 @@ int refine_geom(lj_context* ctx, const GridGeom& g, const lj_params* p, int Kx, 
     return LJ_OK;
 }
 
-// Shape of a call.  Refinement: an explicit p->refine wins; otherwise the half-edge
-// grid when the cell_edge grid averages >= 2 particles per cell and the refined grid
-// stays within 2^28 cells (the CUDALJ mode keeps the reference's 27 cells).  Block 256
-// and no queue unless p says otherwise (measured on MI355X at the C3 density: the
-// queued chain's extra LDS and dependent partner loads cost more than its lane
-// utilisation gains).
+// Shape of a call: explicit fields of p win.  Default grid: cells of rc/2 along y and
+// z and rc/4 along x for large dense inputs (>= 8 particles per cell_edge cell and
+// >= 2^22 particles, where fewer candidates pay for the larger key and cell arrays),
+// the cell_edge grid otherwise (small inputs, and sparse ones whose extra empty cells
+// cost more in the scan than they save; measured on MI355X at 0.1M-16.7M particles).
+// The CUDALJ mode keeps the reference's 27 cells.  Block 256 and no queue.
 Shape choose_shape(const lj_params* p, int64_t n, uint64_t ncells) {
     Shape sh;
     if (p->predicate == LJ_PRED_CUDALJ) {
         sh.refine = sh.refine_x = 1;
     } else {
-        if (p->refine == 1 || p->refine == 2 || p->refine == 4) sh.refine = p->refine;
-        else {
-            const double ppc = (double)n / (double)(ncells ? ncells : 1);
-            sh.refine = (ppc >= 2.0 && (double)ncells * 8.0 < (double)(1u << 28)) ? 2 : 1;
-        }
-        sh.refine_x = p->refine_x ? p->refine_x : sh.refine;
+        const double ppc = (double)n / (double)(ncells ? ncells : 1);
+        const bool dense = ppc >= 8.0 && n >= ((int64_t)1 << 22) && (double)ncells * 16.0 < (double)(1u << 30);
+        sh.refine = (p->refine == 1 || p->refine == 2 || p->refine == 4) ? p->refine : (dense ? 2 : 1);
+        sh.refine_x = p->refine_x ? p->refine_x : (p->refine ? sh.refine : (dense ? 4 : 1));
         if (sh.refine_x < sh.refine) sh.refine_x = sh.refine;
     }
     sh.block = p->block ? p->block : 256;
