@@ -96,7 +96,7 @@ template <int PRED>
 __global__ __launch_bounds__(256) void k_pair_count(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
                                                     const uint32_t* __restrict__ off, uint32_t n, GridGeom g,
                                                     PairConst c, uint32_t* __restrict__ rowcnt) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float4 pi = pos[k];
     int32_t cx, cy, cz;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_pair_fill(const float4* __restrict__ po
                                                    const uint32_t* __restrict__ off, uint32_t n, GridGeom g,
                                                    PairConst c, const unsigned long long* __restrict__ rows,
                                                    uint32_t* __restrict__ cols) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float4 pi = pos[k];
     const uint32_t i = __float_as_uint(pi.w);

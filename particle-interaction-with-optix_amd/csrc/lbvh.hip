@@ -268,7 +268,9 @@ __global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ l
                                                     float* __restrict__ fz, float* __restrict__ e,
                                                     double* __restrict__ partials, lj_result* __restrict__ res,
                                                     int* __restrict__ overflow) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    // consecutive queries walk nearly the same nodes: keep them on one XCD (own L2)
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int k = (int)(blk * blockDim.x + threadIdx.x);
     PairSums<PREC, false, VIR> acc;
     unsigned long long np = 0, tests = 0;
     double esum = 0.0, vsum = 0.0;
@@ -296,8 +298,8 @@ __global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ l
     const unsigned long long bp = block_sum_u64(np);
     const unsigned long long bt = block_sum_u64(tests);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = esum;
-        partials[2 * blockIdx.x + 1] = vsum;
+        partials[2 * blk] = esum;
+        partials[2 * blk + 1] = vsum;
         if (bp) atomicAdd(&res->n_pairs, bp);
         if (bt) atomicAdd(&res->n_candidates, bt);
     }
@@ -307,7 +309,7 @@ template <int PRED>
 __global__ __launch_bounds__(256) void k_lbvh_count(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                                     const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
                                                     uint32_t* __restrict__ rowcnt, int* __restrict__ overflow) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
     if (k >= n) return;
     const float4 pi = leaf[k];
     uint32_t cnt = 0;
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(256) void k_lbvh_fill(const float4* __restrict__ le
                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
                                                    const unsigned long long* __restrict__ rows,
                                                    uint32_t* __restrict__ cols, int* __restrict__ overflow) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
     if (k >= n) return;
     const float4 pi = leaf[k];
     const uint32_t i = __float_as_uint(pi.w);
