@@ -179,13 +179,13 @@ static void release_all(lj_context* ctx) {
         }
     BinWork& b = ctx->bin;
     for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.hist, &b.scan_tmp, &b.counts, &b.offsets, &b.pos, &b.pos_alt,
-                      &b.cell_of, &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets})
+                      &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets})
         d->release();
     LbvhWork& l = ctx->lbvh;
     for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags})
         d->release();
     BatchWork& t = ctx->batch;
-    for (DevBuf* d : {&t.sys, &t.cell_base, &t.part_sys, &t.tiles, &t.partials, &t.results_tmp}) d->release();
+    for (DevBuf* d : {&t.sys, &t.npair, &t.ev}) d->release();
     ctx->tune_out.release();
 }
 

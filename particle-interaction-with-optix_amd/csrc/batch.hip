@@ -236,8 +236,8 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     const int64_t n = (int64_t)ntot;
     const uint32_t ncells = (uint32_t)ctot, nccells = (uint32_t)cctot;
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    LJ_HIP(ctx, W.partials.grow(nn * sizeof(double2)));
-    LJ_HIP(ctx, W.part_sys.grow(nn * sizeof(uint32_t)));
+    LJ_HIP(ctx, W.ev.grow(nn * sizeof(double2)));
+    LJ_HIP(ctx, W.npair.grow(nn * sizeof(uint32_t)));
     uint32_t* off;
     {
         PhaseTimer tb(ctx, LJ_PHASE_BIN, s);
@@ -264,8 +264,8 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(nccells, 256)), dim3(256), 0, s, coff, nccells, dsys,
                            dccbase, (uint32_t)nsys, results);
     }
-    double2* ev = W.partials.as<double2>();
-    uint32_t* npair = W.part_sys.as<uint32_t>();
+    double2* ev = W.ev.as<double2>();
+    uint32_t* npair = W.npair.as<uint32_t>();
     const float4* pos = B.pos.as<float4>();
     const uint32_t* cell_of = B.sorted_keys();
     {

@@ -8,8 +8,9 @@
 // analytic gradient of the same potential, and per-particle outputs are written
 // in the caller's order.
 //
-// Cell order is z-major (cudaLJ.cu:127-134), so the three x-cells of one (y,z) row
-// of the stencil are one contiguous slot range: the 27 cells are 9 ranges.
+// Cell order is z-major (cudaLJ.cu:127-134), so the x-cells of one (y,z) row of the
+// stencil are one contiguous slot range: the 27 cells are 9 ranges (the stencil loop
+// itself is stencil_pairs in lj_stencil.h).
 #include <cmath>
 
 #include "lj_stencil.h"
@@ -19,14 +20,13 @@ namespace lj {
 namespace {
 
 // One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues in
-// dynamic LDS (depth x block words; none without QUEUE).  Outputs go to the caller's order (pi.w holds the
-// original index); per-block FP64 energy and virial partials are reduced later in a
-// fixed order.
+// dynamic LDS (depth x block words; none without QUEUE).  Outputs go to the caller's
+// order (pi.w holds the original index); per-block FP64 energy and virial partials are
+// reduced later in a fixed order.
 template <int PRED, int PREC, bool VIR, bool QUEUE>
 __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
                                                      const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int Rx,
-                                                     int R,
-                                                     Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
+                                                     int R, Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
                                                      float* __restrict__ fy, float* __restrict__ fz,
                                                      float* __restrict__ e, double* __restrict__ partials,
                                                      unsigned long long* __restrict__ npairs) {
@@ -37,8 +37,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ 
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR> acc;
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
-    const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u, Rx, R, tr,
-                                                       c, queue + threadIdx.x, blockDim.x, depth, acc);
+    const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u,
+                                                              Rx, R, tr, c, queue + threadIdx.x, blockDim.x, depth, acc);
     double esum = 0.0, vsum = 0.0;
     if (active) {
         const uint32_t i = __float_as_uint(pi.w);

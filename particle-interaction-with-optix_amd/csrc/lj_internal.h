@@ -57,7 +57,6 @@ struct BinWork {
     DevBuf offsets;      // u32 per-cell offsets (ncells+1)
     DevBuf pos;          // float4 sorted positions, w = original index bits
     DevBuf pos_alt;      // float4 ping-pong partner of pos during the sort
-    DevBuf cell_of;      // u32 cell key per sorted slot
     DevBuf partials;     // per-block partial sums (double) for deterministic totals
     DevBuf counters;     // u64 counters
     DevBuf rows;         // u64 pair-list row offsets
@@ -79,12 +78,9 @@ struct LbvhWork {
 };
 
 struct BatchWork {
-    DevBuf sys;           // device copy of per-system descriptors
-    DevBuf cell_base;     // u32 first global cell of each system
-    DevBuf part_sys;      // u32 system of each particle
-    DevBuf tiles;         // work items
-    DevBuf partials;
-    DevBuf results_tmp;
+    DevBuf sys;           // device copy of the plan: per-system descriptors and cell bases
+    DevBuf npair;         // u32 accepted-pair count of each particle (caller order)
+    DevBuf ev;            // double2 energy and virial of each particle (caller order)
     std::vector<uint8_t> plan;  // host copy of the plan last uploaded to sys
 };
 

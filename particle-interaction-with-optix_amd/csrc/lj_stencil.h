@@ -87,18 +87,18 @@ __device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const
 // The stencil is (2Rx+1) x (2R+1)^2 cells of g, visited as (2R+1)^2 x-rows in z -> y
 // order, rows and x-cells beyond the cutoff trimmed.
 //
-// The candidate loop only evaluates the predicate.  An accepted partner's slot goes
-// to the lane's queue in LDS (entry q of the lane at queue[q * stride]: lanes of a
-// wave store to consecutive words, never a bank conflict), and the pair arithmetic
-// runs when some lane's queue is nearly full, on every lane's queue at once.
-// Evaluated inside the candidate loop, the FP64 pair chain would run for the whole
-// wave whenever any one lane accepts (a quarter of the lanes at the cutoff densities
-// of interest); from the queues it runs with most lanes busy.  Queues are emptied in
-// order, so a particle's sum keeps the stencil order of its partners whatever the
-// queue depth.  With QUEUE = false the pair arithmetic runs inside the candidate loop
-// instead (same order, same bits; faster when nearly every lane accepts).  Every lane
-// of the wave must call this (inactive lanes with active = false), because the
-// flushes are wave-wide.
+// Pairs.  With QUEUE = false (the default shape) the FP64 pair arithmetic of an
+// accepted candidate runs inside the candidate loop; it then runs for the whole wave
+// whenever any lane accepts, with about a quarter of the lanes busy.  With QUEUE =
+// true the candidate loop only evaluates the predicate: an accepted partner's slot
+// goes to the lane's queue in LDS (entry q of the lane at queue[q * stride]: lanes of
+// a wave store to consecutive words, never a bank conflict), and the pair arithmetic
+// runs when some lane's queue is nearly full, on every lane's queue at once, in queue
+// order (same order, same bits).  On MI355X the queue was slower on every workload
+// measured -- lanes fill their queues at different rates, a flush runs as long as the
+// fullest queue, and it reloads the queued partners -- so it is a tuning option, not
+// the default.  Every lane of the wave must call this (inactive lanes with active =
+// false), because the flushes are wave-wide.
 template <int PRED, int PREC, bool VIR, bool QUEUE>
 __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
                                          bool active, const float4& pi, const GridGeom& g, uint32_t cbase,
