@@ -222,12 +222,22 @@ __device__ inline bool overlaps(float4 p, float h, float4 lo, float4 hi) {
 // Visits every leaf whose position lies within the query box of leaf k, calling
 // f(slot, leaf position) for each one except k itself.  Depth-first, without a stack:
 // on overlap go to the left child, otherwise (and after a leaf or bucket) follow the
-// rope.  Returns false if the walk exceeds 2n steps (impossible for a well-formed
-// tree, but checked).
+// rope.  The walk runs in two phases so that a wave does the same kind of work at the
+// same time: (1) walk the internal nodes and list, in LDS, the buckets and lone leaves
+// that meet the query box, until the walk ends or the list holds kList entries; (2)
+// scan the listed leaves in list order; repeat while the walk is unfinished (it stops
+// at a node and resumes there, having no stack).  Interleaving the two made every wave
+// step pay for a bucket scan as soon as one lane needed one.  The leaves are visited
+// in walk order either way.  Returns false if the walk exceeds 2n steps (impossible
+// for a well-formed tree, but checked).  list: this lane's entries, at list[q * stride].
+constexpr int kList = 32;
+constexpr int kLbvhBlock = 128;
+
 template <class F>
 __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                 const float4* __restrict__ box, const int* __restrict__ rope_by_end, int n, int k,
-                                float4 p, float h, unsigned long long& tests, F&& f) {
+                                float4 p, float h, unsigned long long& tests, int* __restrict__ list, uint32_t stride,
+                                F&& f) {
     if (n == 1) return true;
     auto leaf_test = [&](int slot) {
         const float4 pj = leaf[slot];
@@ -239,35 +249,53 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
     };
     int node = n <= kBucket ? (0 | kCollapsed) : 0;  // a small tree is one bucket
     const uint32_t limit = 2u * (uint32_t)n;
-    for (uint32_t steps = 0; node != kSentinel; ++steps) {
-        if (steps > limit) return false;
-        if (node < 0) {  // bucket
-            const int b = node & 0x7fffffff;
-            const float4 lo = box[2 * b], hi = box[2 * b + 1];
-            if (overlaps(p, h, lo, hi)) {
-                const int2 r = range[b];
+    uint32_t steps = 0;
+    bool ok = true;
+    while (node != kSentinel) {
+        // phase 1: entries are bucket nodes (< n - 1) or lone leaves (node ids >= n - 1)
+        unsigned cnt = 0;
+        while (node != kSentinel && cnt < (unsigned)kList) {
+            if (++steps > limit) {
+                ok = false;
+                node = kSentinel;
+                break;
+            }
+            if (node < 0) {  // bucket
+                const int b = node & 0x7fffffff;
+                const float4 lo = box[2 * b], hi = box[2 * b + 1];
+                if (overlaps(p, h, lo, hi)) list[(cnt++) * stride] = b;
+                node = __float_as_int(hi.w);
+            } else if (node >= n - 1) {  // a leaf that is not in a bucket
+                list[(cnt++) * stride] = node;
+                node = rope_by_end[node - (n - 1)];
+            } else {
+                const float4 lo = box[2 * node], hi = box[2 * node + 1];
+                node = overlaps(p, h, lo, hi) ? __float_as_int(lo.w) : __float_as_int(hi.w);
+            }
+        }
+        // phase 2
+        for (unsigned q = 0; q < cnt; ++q) {
+            const int e = list[q * stride];
+            if (e >= n - 1) {
+                leaf_test(e - (n - 1));
+            } else {
+                const int2 r = range[e];
                 for (int slot = r.x; slot <= r.y; ++slot) leaf_test(slot);
             }
-            node = __float_as_int(hi.w);
-        } else if (node >= n - 1) {  // a leaf that is not in a bucket
-            const int slot = node - (n - 1);
-            leaf_test(slot);
-            node = rope_by_end[slot];
-        } else {
-            const float4 lo = box[2 * node], hi = box[2 * node + 1];
-            node = overlaps(p, h, lo, hi) ? __float_as_int(lo.w) : __float_as_int(hi.w);
         }
     }
-    return true;
+    return ok;
 }
 
 template <int PRED, int PREC, bool VIR>
-__global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ leaf, const int2* __restrict__ range,
+__global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                                     const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
                                                     float* __restrict__ fx, float* __restrict__ fy,
                                                     float* __restrict__ fz, float* __restrict__ e,
                                                     double* __restrict__ partials, lj_result* __restrict__ res,
                                                     int* __restrict__ overflow) {
+    extern __shared__ int lds_list[];
+    int* list = lds_list + threadIdx.x;
     // consecutive queries walk nearly the same nodes: keep them on one XCD (own L2)
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int k = (int)(blk * blockDim.x + threadIdx.x);
@@ -277,7 +305,7 @@ __global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ l
     if (k < n) {
         const float4 pi = leaf[k];
         acc.set_i(pi);
-        const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, [&](int, float4 pj) {
+        const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, list, blockDim.x, [&](int, float4 pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float d2 = dist2(dx, dy, dz);
             if (!accept<PRED>(c, d2)) return;
@@ -306,15 +334,17 @@ __global__ __launch_bounds__(256) void k_lbvh_force(const float4* __restrict__ l
 }
 
 template <int PRED>
-__global__ __launch_bounds__(256) void k_lbvh_count(const float4* __restrict__ leaf, const int2* __restrict__ range,
+__global__ __launch_bounds__(kLbvhBlock) void k_lbvh_count(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                                     const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
                                                     uint32_t* __restrict__ rowcnt, int* __restrict__ overflow) {
+    extern __shared__ int lds_list[];
+    int* list = lds_list + threadIdx.x;
     const int k = (int)(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
     if (k >= n) return;
     const float4 pi = leaf[k];
     uint32_t cnt = 0;
     unsigned long long tests = 0;
-    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, [&](int, float4 pj) {
+    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, list, blockDim.x, [&](int, float4 pj) {
         if (accept<PRED>(c, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z))) ++cnt;
     });
     if (!ok) atomicOr(overflow, 1);
@@ -322,10 +352,12 @@ __global__ __launch_bounds__(256) void k_lbvh_count(const float4* __restrict__ l
 }
 
 template <int PRED>
-__global__ __launch_bounds__(256) void k_lbvh_fill(const float4* __restrict__ leaf, const int2* __restrict__ range,
+__global__ __launch_bounds__(kLbvhBlock) void k_lbvh_fill(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
                                                    const unsigned long long* __restrict__ rows,
                                                    uint32_t* __restrict__ cols, int* __restrict__ overflow) {
+    extern __shared__ int lds_list[];
+    int* list = lds_list + threadIdx.x;
     const int k = (int)(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
     if (k >= n) return;
     const float4 pi = leaf[k];
@@ -333,7 +365,7 @@ __global__ __launch_bounds__(256) void k_lbvh_fill(const float4* __restrict__ le
     const unsigned long long r0 = rows[i], r1 = rows[i + 1];
     unsigned long long w = r0;
     unsigned long long tests = 0;
-    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, [&](int, float4 pj) {
+    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, list, blockDim.x, [&](int, float4 pj) {
         if (w < r1 && accept<PRED>(c, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z))) cols[w++] = __float_as_uint(pj.w);
     });
     if (!ok) atomicOr(overflow, 1);
@@ -444,7 +476,8 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
         res = ctx->bin.counters.as<lj_result>();
     }
     fill_u64(&res->n_candidates, 2, 0ull, s);
-    const unsigned nb = div_up(n > 0 ? n : 1, 256);
+    const unsigned nb = div_up(n > 0 ? n : 1, kLbvhBlock);
+    const size_t lds = (size_t)kList * kLbvhBlock * sizeof(int);
     LJ_HIP(ctx, ctx->bin.partials.grow(((size_t)nb + nb / 1024 + 1) * 2 * sizeof(double)));
     double* partials = ctx->bin.partials.as<double>();
     int* ovf = overflow_flag(ctx, n);
@@ -455,7 +488,7 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     const float4* box = ctx->lbvh.node_box.as<float4>();
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
 #define LJ_LB(PRED, PREC, VIR)                                                                                        \
-    hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(256), 0, s, leaf, range, rope, box, (int)n, h, c, fx, \
+    hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, c, fx, \
                        fy, fz, e, partials, res, ovf)
 #define LJ_LB_PREC(PRED)                                                                                    \
     switch (p->precision) {                                                                                 \
@@ -472,7 +505,7 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
         case LJ_PRED_TRIANGLE: LJ_LB_PREC(LJ_PRED_TRIANGLE); break;
         }
     } else {
-        hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true>), dim3(1), dim3(256), 0, s, leaf, range, rope, box,
+        hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true>), dim3(1), dim3(kLbvhBlock), lds, s, leaf, range, rope, box,
                            0, h, c, fx, fy, fz, e, partials, res, ovf);
     }
 #undef LJ_LB_PREC
@@ -495,7 +528,8 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     LJ_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)stream;
     LJ_TRY(lbvh_build(ctx, x, y, z, n, p, s));
-    const unsigned nb = div_up(n > 0 ? n : 1, 256);
+    const unsigned nb = div_up(n > 0 ? n : 1, kLbvhBlock);
+    const size_t lds = (size_t)kList * kLbvhBlock * sizeof(int);
     LJ_HIP(ctx, ctx->bin.rows.grow(((size_t)n + 1) * sizeof(uint32_t)));
     uint32_t* rowcnt = ctx->bin.rows.as<uint32_t>();
     int* ovf = overflow_flag(ctx, n);
@@ -506,7 +540,7 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     const float4* box = ctx->lbvh.node_box.as<float4>();
     if (n > 0) {
         switch (p->predicate) {
-#define LJ_C(P) case P: hipLaunchKernelGGL(k_lbvh_count<P>, dim3(nb), dim3(256), 0, s, leaf, range, rope, box, (int)n, h, c, rowcnt, ovf); break;
+#define LJ_C(P) case P: hipLaunchKernelGGL(k_lbvh_count<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, c, rowcnt, ovf); break;
             LJ_C(LJ_PRED_AABB) LJ_C(LJ_PRED_CUSTOM) LJ_C(LJ_PRED_SPHERE) LJ_C(LJ_PRED_TRIANGLE)
 #undef LJ_C
         }
@@ -522,7 +556,7 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     if (total > capacity) return fail(ctx, LJ_ERR_CAPACITY, "pair list needs %llu entries, capacity is %llu", total, capacity);
     if (total == 0) return LJ_OK;
     switch (p->predicate) {
-#define LJ_F(P) case P: hipLaunchKernelGGL(k_lbvh_fill<P>, dim3(nb), dim3(256), 0, s, leaf, range, rope, box, (int)n, h, c, (const unsigned long long*)row_offsets, cols, ovf); break;
+#define LJ_F(P) case P: hipLaunchKernelGGL(k_lbvh_fill<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, c, (const unsigned long long*)row_offsets, cols, ovf); break;
         LJ_F(LJ_PRED_AABB) LJ_F(LJ_PRED_CUSTOM) LJ_F(LJ_PRED_SPHERE) LJ_F(LJ_PRED_TRIANGLE)
 #undef LJ_F
     }
