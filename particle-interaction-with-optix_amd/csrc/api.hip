@@ -178,7 +178,7 @@ static void release_all(lj_context* ctx) {
             (void)hipEventDestroy(e.second);
         }
     BinWork& b = ctx->bin;
-    for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.hist, &b.scan_tmp, &b.counts, &b.offsets, &b.pos, &b.pos_alt,
+    for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.hist, &b.scan_tmp, &b.gaps, &b.offsets, &b.pos, &b.pos_alt,
                       &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets})
         d->release();
     LbvhWork& l = ctx->lbvh;

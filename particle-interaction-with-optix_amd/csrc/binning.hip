@@ -1,6 +1,6 @@
-// binning.hip -- cell binning on gfx950: cell keys + histogram, device-wide scan,
-// stable LSD radix sort of (cell key, original index), gather of the sorted
-// positions, and the 27-cell candidate count.
+// binning.hip -- cell binning on gfx950: stable LSD radix sort of (cell key, original
+// index) carrying the positions, cell offsets from the sorted keys, device-wide scans,
+// and the 27-cell candidate count.
 //
 // Reference: ComputeNbParticlePerCells (Cuda-version/cudaLJ.cu:345-383),
 // PrefixFullV2 / BuildPartialPrefix_AllSize / ApplyPartialPrefixV3
@@ -15,6 +15,9 @@
 //  - the scatter is a stable radix sort, so the order inside a cell is the input
 //    order and every output is deterministic (the reference hands out slots with
 //    atomic tickets, cudaLJ.cu:462);
+//  - the cell offsets come from the sorted keys in one pass that writes every cell
+//    once, with no per-cell count array or scan (the reference counts with atomics
+//    and prefix-sums the counts, cudaLJ.cu:345-383 and prefixsum.hpp);
 //  - the candidate count is a closed form over cells, not a pass over particles.
 #include "lj_internal.h"
 #include "lj_sort.h"
@@ -100,31 +103,57 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI* in, TS* ou
     if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) *total = run;
 }
 
-// ---------------------------------------------------------------- cell counts
-// Number of particles of each occupied cell from the sorted keys: the first slot of a
-// run gallops to the run's end (O(log run length)); empty cells keep the zero fill.
-__global__ __launch_bounds__(256) void k_run_counts(const uint32_t* __restrict__ skeys, uint32_t n,
-                                                    uint32_t* __restrict__ counts) {
+// ---------------------------------------------------------------- cell offsets
+// off[c] = number of sorted keys below c, c in [0, ncells], straight from the sorted
+// keys: the first slot k of each run of equal keys writes k into the cells above the
+// previous run's key up to its own key, and the last slot writes n into the cells above
+// the last key.  Every cell is written exactly once, so no fill, per-cell counts or
+// scan are needed.  A gap of more than kGapInline cells is cut into pieces of at most
+// kGapPiece cells, appended to a list that k_fill_gaps spreads over whole workgroups
+// (sparse grids: one lane never loops over a long empty stretch).
+constexpr uint32_t kGapInline = 256;
+constexpr uint32_t kGapPiece = 4096;
+constexpr unsigned kGapBlocks = 1024;
+
+struct GapPiece {
+    uint32_t a, b, v;  // off[a .. b) = v
+};
+
+__device__ inline void fill_cells(uint32_t a, uint32_t b, uint32_t v, uint32_t* __restrict__ off,
+                                  GapPiece* __restrict__ gaps, uint32_t* __restrict__ ngaps) {
+    if (b - a <= kGapInline) {
+        for (uint32_t c = a; c < b; ++c) off[c] = v;
+        return;
+    }
+    const uint32_t pieces = (b - a + kGapPiece - 1) / kGapPiece;
+    const uint32_t at = atomicAdd(ngaps, pieces);
+    for (uint32_t i = 0; i < pieces; ++i) {
+        const uint32_t pa = a + i * kGapPiece;
+        gaps[at + i] = GapPiece{pa, b - pa > kGapPiece ? pa + kGapPiece : b, v};
+    }
+}
+
+__global__ __launch_bounds__(256) void k_offsets(const uint32_t* __restrict__ skeys, uint32_t n, uint32_t ncells,
+                                                 uint32_t* __restrict__ off, GapPiece* __restrict__ gaps,
+                                                 uint32_t* __restrict__ ngaps) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t key = skeys[k];
-    if (k > 0 && skeys[k - 1] == key) return;
-    uint32_t lo = k, hi = n, step = 1;  // skeys[lo] == key; hi == n or skeys[hi] != key
-    while (true) {
-        const uint32_t probe = lo + step;
-        if (probe >= n || skeys[probe] != key) {
-            hi = probe < n ? probe : n;
-            break;
-        }
-        lo = probe;
-        step <<= 1;
+    if (k == 0) fill_cells(0u, key + 1u, 0u, off, gaps, ngaps);
+    else {
+        const uint32_t prev = skeys[k - 1];
+        if (prev != key) fill_cells(prev + 1u, key + 1u, k, off, gaps, ngaps);
     }
-    while (hi - lo > 1) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (skeys[mid] == key) lo = mid;
-        else hi = mid;
+    if (k == n - 1) fill_cells(key + 1u, ncells + 1u, n, off, gaps, ngaps);
+}
+
+__global__ __launch_bounds__(256) void k_fill_gaps(const GapPiece* __restrict__ gaps,
+                                                   const uint32_t* __restrict__ ngaps, uint32_t* __restrict__ off) {
+    const uint32_t m = *ngaps;
+    for (uint32_t e = blockIdx.x; e < m; e += gridDim.x) {
+        const GapPiece p = gaps[e];
+        for (uint32_t c = p.a + threadIdx.x; c < p.b; c += blockDim.x) off[c] = p.v;
     }
-    counts[key] = hi - k;
 }
 
 __global__ __launch_bounds__(256) void k_perm_from_pos(const float4* __restrict__ pos, uint32_t n,
@@ -244,15 +273,27 @@ int scan_u32_to_u64(lj_context* ctx, const uint32_t* in, unsigned long long* out
 
 int cell_offsets_from_sorted(lj_context* ctx, int64_t n, uint32_t ncells, hipStream_t s) {
     BinWork& w = ctx->bin;
-    LJ_HIP(ctx, w.counts.grow((size_t)ncells * 4));
     LJ_HIP(ctx, w.offsets.grow(((size_t)ncells + 1) * 4));
-    uint32_t* counts = w.counts.as<uint32_t>();
-    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(ncells, 256)), dim3(256), 0, s, counts, (uint64_t)ncells, 0u);
-    if (n > 0)
-        hipLaunchKernelGGL(k_run_counts, dim3(div_up(n, 256)), dim3(256), 0, s, w.sorted_keys(), (uint32_t)n, counts);
-    // offsets[0..nc) = exclusive scan of counts, offsets[nc] = n
     uint32_t* off = w.offsets.as<uint32_t>();
-    return scan_u32(ctx, counts, off, ncells, off + ncells, s);
+    if (n <= 0) {
+        fill_u32(off, (uint64_t)ncells + 1, 0u, s);
+        LJ_HIP(ctx, hipGetLastError());
+        return LJ_OK;
+    }
+    // gaps are disjoint and each piece list entry covers kGapPiece cells, or ends a gap
+    // of more than kGapInline cells
+    const size_t cells = (size_t)ncells + 1;
+    const size_t cap = cells / kGapPiece + cells / kGapInline + 2;
+    LJ_HIP(ctx, w.gaps.grow(sizeof(uint32_t) * 4 + cap * sizeof(GapPiece)));
+    uint32_t* ngaps = w.gaps.as<uint32_t>();
+    GapPiece* gaps = reinterpret_cast<GapPiece*>(w.gaps.as<uint8_t>() + sizeof(uint32_t) * 4);
+    fill_u32(ngaps, 1, 0u, s);
+    hipLaunchKernelGGL(k_offsets, dim3(div_up(n, 256)), dim3(256), 0, s, w.sorted_keys(), (uint32_t)n, ncells, off,
+                       gaps, ngaps);
+    const unsigned nb = (unsigned)(cap < kGapBlocks ? cap : kGapBlocks);
+    hipLaunchKernelGGL(k_fill_gaps, dim3(nb), dim3(256), 0, s, (const GapPiece*)gaps, (const uint32_t*)ngaps, off);
+    LJ_HIP(ctx, hipGetLastError());
+    return LJ_OK;
 }
 
 int bin_particles(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,

@@ -53,7 +53,7 @@ struct BinWork {
     DevBuf keys[2];      // u32 radix keys (ping-pong)
     DevBuf hist;         // radix digit histograms
     DevBuf scan_tmp;     // partial sums of the device-wide scans
-    DevBuf counts;       // u32 per-cell counts
+    DevBuf gaps;         // count + list of long empty-cell stretches for the cell offsets
     DevBuf offsets;      // u32 per-cell offsets (ncells+1)
     DevBuf pos;          // float4 sorted positions, w = original index bits
     DevBuf pos_alt;      // float4 ping-pong partner of pos during the sort

@@ -1,13 +1,17 @@
 // lj_sort.h -- stable LSD radix sort of particles by a 32-bit key, carrying the payload
 // (x, y, z, original index) through every pass.
 //
-// One pass over 8 key bits is two kernels around a device-wide scan:
-//   histogram  per 2048-key tile, digit-major table hist[d * ntiles + tile]
-//   scan       exclusive scan of the table: hist[d, tile] = first output slot
-//   scatter    each tile ranks its keys stably (input order inside a digit), stages
-//              key + payload in LDS in digit order, and writes every digit's run to
-//              consecutive output slots, so stores are contiguous runs rather than
-//              one scattered 20-byte record per lane.
+// One pass over 8 key bits is:
+//   histogram  per 2048-key tile, tile-major table hist[tile * 256 + d] (one
+//              contiguous 1 KB row per tile)
+//   offsets    a digit-major exclusive scan of that table in three small kernels:
+//              hist[tile * 256 + d] becomes the first output slot of digit d's keys
+//              in that tile
+//   scatter    each wave ranks its own contiguous run of keys stably with per-wave LDS
+//              counters (no block barrier per round), the waves' counts are combined
+//              once, the tile is staged in LDS in digit order, and every digit's run
+//              goes to consecutive output slots, so stores are contiguous runs rather
+//              than one scattered 20-byte record per lane.
 // The first pass computes the keys from the positions with a key functor (cell key,
 // batched cell key, Morton code) and reads x, y, z in input order, so no pass ever
 // gathers positions through a permutation.
@@ -22,10 +26,17 @@ namespace sortk {
 constexpr int kWave = 64;
 constexpr int kBits = 8;
 constexpr int kRadix = 1 << kBits;
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;  // histogram
 constexpr int kItems = 8;
 constexpr int kTile = kThreads * kItems;  // 2048 keys per tile
-constexpr int kWaves = kThreads / kWave;
+// scatter: more, shorter waves per tile than the histogram, so that more waves share a
+// CU while the tile's staging area (which bounds the workgroups per CU) stays the same
+constexpr int kSThreads = 1024;
+constexpr int kSItems = kTile / kSThreads;
+constexpr int kSWaves = kSThreads / kWave;
+constexpr int kWaveKeys = kSItems * kWave;  // the contiguous keys of one wave in a tile
+static_assert(kSThreads >= kRadix && kSItems * kSThreads == kTile, "scatter shape");
+constexpr uint32_t kMaxChunks = 256;       // row groups of the offset scan
 
 __device__ inline unsigned rank_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -49,8 +60,7 @@ __device__ inline void load_elem(uint32_t idx, const float* __restrict__ x, cons
 template <bool FIRST, class KeyOp>
 __global__ __launch_bounds__(kThreads) void k_hist(const float* __restrict__ x, const float* __restrict__ y,
                                                    const float* __restrict__ z, const uint32_t* __restrict__ kin,
-                                                   uint32_t n, KeyOp op, int shift, uint32_t ntiles,
-                                                   uint32_t* __restrict__ hist) {
+                                                   uint32_t n, KeyOp op, int shift, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kRadix];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -66,64 +76,121 @@ __global__ __launch_bounds__(kThreads) void k_hist(const float* __restrict__ x, 
         }
     }
     __syncthreads();
-    hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    hist[(uint64_t)blockIdx.x * kRadix + threadIdx.x] = h[threadIdx.x];
 }
 
+// ---- digit-major exclusive scan of the tile-major table, rows grouped in chunks of
+// `chunk` consecutive tiles (at most kMaxChunks chunks).
+// part[c * 256 + d] = sum of digit d over the rows of chunk c.
+static __global__ __launch_bounds__(kRadix) void k_dig_reduce(const uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                       uint32_t chunk, uint32_t* __restrict__ part) {
+    const uint32_t t0 = blockIdx.x * chunk;
+    const uint32_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
+    uint32_t s = 0;
+#pragma unroll 8
+    for (uint32_t t = t0; t < t1; ++t) s += hist[(uint64_t)t * kRadix + threadIdx.x];
+    part[blockIdx.x * kRadix + threadIdx.x] = s;
+}
+
+// One block of 4 x 256 threads; thread (q, d) owns a quarter of the chunks of digit d.
+// part[c * 256 + d] becomes the first output slot of digit d's keys in chunk c:
+// (keys of smaller digits) + (keys of digit d in earlier chunks).
+static __global__ __launch_bounds__(4 * kRadix) void k_dig_scan(uint32_t* __restrict__ part, uint32_t nchunks) {
+    __shared__ uint32_t qsum[4][kRadix];
+    __shared__ uint32_t dbase[kRadix];
+    __shared__ uint32_t wsum[kRadix / kWave];
+    const unsigned d = threadIdx.x % kRadix, q = threadIdx.x / kRadix;
+    const uint32_t per = (nchunks + 3) / 4;
+    const uint32_t c0 = q * per < nchunks ? q * per : nchunks;
+    const uint32_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
+    uint32_t s = 0;
+#pragma unroll 8
+    for (uint32_t c = c0; c < c1; ++c) s += part[c * kRadix + d];
+    qsum[q][d] = s;
+    __syncthreads();
+    uint32_t tot = 0, incl = 0;
+    if (q == 0) {  // threads 0..255 (waves 0-3): inclusive scan of the digit totals
+        tot = qsum[0][d] + qsum[1][d] + qsum[2][d] + qsum[3][d];
+        incl = tot;
+#pragma unroll
+        for (int k = 1; k < kWave; k <<= 1) {
+            const uint32_t o = __shfl_up(incl, k, kWave);
+            if (__lane_id() >= (unsigned)k) incl += o;
+        }
+        if (__lane_id() == kWave - 1) wsum[d / kWave] = incl;
+    }
+    __syncthreads();
+    if (q == 0) {
+        uint32_t ex = incl - tot;
+        for (unsigned w = 0; w < d / kWave; ++w) ex += wsum[w];
+        dbase[d] = ex;
+    }
+    __syncthreads();
+    uint32_t run = dbase[d];
+    for (unsigned k = 0; k < q; ++k) run += qsum[k][d];
+#pragma unroll 8
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t v = part[c * kRadix + d];
+        part[c * kRadix + d] = run;
+        run += v;
+    }
+}
+
+// hist[t * 256 + d] <- first output slot of digit d's keys in tile t.
+static __global__ __launch_bounds__(kRadix) void k_dig_down(uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t chunk,
+                                                     const uint32_t* __restrict__ part) {
+    const uint32_t t0 = blockIdx.x * chunk;
+    const uint32_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
+    uint32_t run = part[blockIdx.x * kRadix + threadIdx.x];
+#pragma unroll 8
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint64_t at = (uint64_t)t * kRadix + threadIdx.x;
+        const uint32_t v = hist[at];
+        hist[at] = run;
+        run += v;
+    }
+}
+
+// Tile t (logical, XCD-aware: neighbouring tiles write neighbouring output runs and
+// share an L2) sorts its keys stably by digit and writes each digit's run.  Element j
+// of the tile (input order) is handled by wave j / kWaveKeys, round
+// (j % kWaveKeys) / 64, lane j % 64, so (wave, round, lane) is the input order.
 template <bool FIRST, class KeyOp>
-__global__ __launch_bounds__(kThreads) void k_scatter(const float* __restrict__ x, const float* __restrict__ y,
-                                                      const float* __restrict__ z, const uint32_t* __restrict__ kin,
-                                                      const float4* __restrict__ pin, uint32_t n, KeyOp op, int shift,
-                                                      uint32_t ntiles, const uint32_t* __restrict__ hist,
-                                                      uint32_t* __restrict__ kout, float4* __restrict__ pout) {
+__global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__ x, const float* __restrict__ y,
+                                                       const float* __restrict__ z, const uint32_t* __restrict__ kin,
+                                                       const float4* __restrict__ pin, uint32_t n, KeyOp op, int shift,
+                                                       const uint32_t* __restrict__ hist, uint32_t* __restrict__ kout,
+                                                       float4* __restrict__ pout) {
     __shared__ uint32_t skey[kTile];
     __shared__ float4 spay[kTile];
-    __shared__ uint32_t dstart[kRadix];  // first staging slot of each digit in this tile
-    __shared__ uint32_t gbase[kRadix];   // first output slot of each digit for this tile
-    __shared__ uint32_t run[kRadix];     // keys of each digit already ranked in earlier rounds
-    __shared__ uint32_t wcnt[kWaves][kRadix];
-    __shared__ uint32_t wsum[kWaves];
+    __shared__ uint32_t wcnt[kSWaves][kRadix];  // per-wave digit counts, then per-wave first slots
+    __shared__ uint32_t dstart[kRadix];         // first staging slot of each digit in this tile
+    __shared__ uint32_t gbase[kRadix];          // first output slot of each digit for this tile
+    __shared__ uint32_t wsum[kRadix / kWave];
     const unsigned tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-    const uint32_t base = blockIdx.x * kTile;
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t base = tile * kTile;
     const uint32_t count = n - base < (uint32_t)kTile ? n - base : (uint32_t)kTile;
 
-    uint32_t key[kItems];
-    float4 pay[kItems];
 #pragma unroll
-    for (int r = 0; r < kItems; ++r) {
-        const uint32_t j = r * kThreads + tid;
+    for (int q = 0; q < kRadix / kWave; ++q) wcnt[wave][q * kWave + lane] = 0;
+    if (tid < (unsigned)kRadix) gbase[tid] = hist[(uint64_t)tile * kRadix + tid];
+
+    uint32_t key[kSItems];
+    float4 pay[kSItems];
+    uint32_t rank[kSItems];
+#pragma unroll
+    for (int r = 0; r < kSItems; ++r) {
+        const uint32_t j = wave * kWaveKeys + r * kWave + lane;
         if (j < count) load_elem<FIRST>(base + j, x, y, z, kin, pin, op, key[r], pay[r]);
         else key[r] = 0;
     }
-    // tile digit counts -> exclusive starts (thread d owns digit d)
-    run[tid] = 0;
-    gbase[tid] = hist[(uint64_t)tid * ntiles + blockIdx.x];
-    dstart[tid] = 0;
-    __syncthreads();
+    // Stable ranks inside the wave: the lowest lane of each digit group adds the group's
+    // size to the wave's counter with an LDS atomic (the wave's own counters, so no
+    // other wave touches them) and hands the old value to its group.
 #pragma unroll
-    for (int r = 0; r < kItems; ++r)
-        if (r * kThreads + tid < count) atomicAdd(&dstart[(key[r] >> shift) & (kRadix - 1)], 1u);
-    __syncthreads();
-    {
-        const uint32_t v = dstart[tid];
-        uint32_t incl = v;
-#pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-            const uint32_t o = __shfl_up(incl, d, kWave);
-            if (lane >= (unsigned)d) incl += o;
-        }
-        if (lane == kWave - 1) wsum[wave] = incl;
-        __syncthreads();
-        uint32_t off = 0;
-        for (unsigned q = 0; q < wave; ++q) off += wsum[q];
-        dstart[tid] = off + incl - v;
-    }
-    // stable ranks, round by round; (round, wave, lane) is the input order
-#pragma unroll
-    for (int r = 0; r < kItems; ++r) {
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
-        __syncthreads();
-        const bool valid = r * kThreads + tid < count;
+    for (int r = 0; r < kSItems; ++r) {
+        const bool valid = wave * kWaveKeys + r * kWave + lane < count;
         const uint32_t d = (key[r] >> shift) & (kRadix - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -133,30 +200,57 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const float* __restrict__ 
             peers &= bit ? bb : ~bb;
         }
         const unsigned rk = rank_below(peers);
-        if (valid && rk == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            uint32_t acc = run[tid];
+        uint32_t before = 0;
+        if (valid && rk == 0) before = atomicAdd(&wcnt[wave][d], (uint32_t)__popcll(peers));
+        const int leader = valid ? __builtin_ctzll(peers) : (int)lane;
+        before = __shfl(before, leader, kWave);
+        rank[r] = before + rk;
+    }
+    __syncthreads();
+    // threads 0..255, one per digit: staging start of the digit (exclusive scan over
+    // digits of the tile's counts) and of each wave's part of it
+    uint32_t c[kSWaves];
+    uint32_t tot = 0, incl = 0;
+    if (tid < (unsigned)kRadix) {
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = wcnt[w][tid];
-                wcnt[w][tid] = acc;
-                acc += c;
-            }
-            run[tid] = acc;
+        for (int w = 0; w < kSWaves; ++w) {
+            c[w] = wcnt[w][tid];
+            tot += c[w];
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t slot = dstart[d] + wcnt[wave][d] + rk;
+        incl = tot;
+#pragma unroll
+        for (int k = 1; k < kWave; k <<= 1) {
+            const uint32_t o = __shfl_up(incl, k, kWave);
+            if (lane >= (unsigned)k) incl += o;
+        }
+        if (lane == kWave - 1) wsum[wave] = incl;
+    }
+    __syncthreads();
+    if (tid < (unsigned)kRadix) {
+        uint32_t run = incl - tot;
+        for (unsigned q = 0; q < wave; ++q) run += wsum[q];
+        dstart[tid] = run;
+#pragma unroll
+        for (int w = 0; w < kSWaves; ++w) {
+            wcnt[w][tid] = run;
+            run += c[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSItems; ++r) {
+        if (wave * kWaveKeys + r * kWave + lane < count) {
+            const uint32_t d = (key[r] >> shift) & (kRadix - 1);
+            const uint32_t slot = wcnt[wave][d] + rank[r];
             skey[slot] = key[r];
             spay[slot] = pay[r];
         }
-        __syncthreads();
     }
+    __syncthreads();
     // write the staged tile: consecutive slots of one digit go to consecutive outputs
 #pragma unroll
-    for (int r = 0; r < kItems; ++r) {
-        const uint32_t p = r * kThreads + tid;
+    for (int r = 0; r < kSItems; ++r) {
+        const uint32_t p = r * kSThreads + tid;
         if (p < count) {
             const uint32_t k = skey[p];
             const uint32_t d = (k >> shift) & (kRadix - 1);
@@ -208,8 +302,12 @@ int sort_particles(lj_context* ctx, const float* x, const float* y, const float*
         return LJ_OK;
     }
     const unsigned ntiles = div_up(n, kTile);
-    LJ_HIP(ctx, w.hist.grow((size_t)kRadix * ntiles * 4));
+    const uint32_t chunk = div_up(ntiles, kMaxChunks);
+    const uint32_t nchunks = div_up(ntiles, chunk);
+    // the table, then the chunk partials
+    LJ_HIP(ctx, w.hist.grow(((size_t)ntiles + kMaxChunks) * kRadix * 4));
     uint32_t* hist = w.hist.as<uint32_t>();
+    uint32_t* part = hist + (size_t)ntiles * kRadix;
     // the last pass must write ctx->bin.pos
     float4* pbuf[2] = {w.pos.as<float4>(), w.pos_alt.as<float4>()};
     int kcur = 0;
@@ -220,21 +318,25 @@ int sort_particles(lj_context* ctx, const float* x, const float* y, const float*
         const float4* psrc = pbuf[(passes - p) & 1];
         if (p == 0) {
             hipLaunchKernelGGL((k_hist<true, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
-                               (const uint32_t*)nullptr, (uint32_t)n, op, shift, ntiles, hist);
+                               (const uint32_t*)nullptr, (uint32_t)n, op, shift, hist);
         } else {
             hipLaunchKernelGGL((k_hist<false, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
-                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), (uint32_t)n, op, shift, ntiles, hist);
+                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), (uint32_t)n, op, shift, hist);
         }
-        LJ_TRY(scan_u32(ctx, hist, hist, (int64_t)kRadix * ntiles, nullptr, s));
+        hipLaunchKernelGGL(k_dig_reduce, dim3(nchunks), dim3(kRadix), 0, s, (const uint32_t*)hist, ntiles, chunk,
+                           part);
+        hipLaunchKernelGGL(k_dig_scan, dim3(1), dim3(4 * kRadix), 0, s, part, nchunks);
+        hipLaunchKernelGGL(k_dig_down, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, chunk,
+                           (const uint32_t*)part);
         if (p == 0) {
             // pass 0 reads positions; it writes keys[1] so that keys[0] stays free
-            hipLaunchKernelGGL((k_scatter<true, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
-                               (const uint32_t*)nullptr, (const float4*)nullptr, (uint32_t)n, op, shift, ntiles,
+            hipLaunchKernelGGL((k_scatter<true, KeyOp>), dim3(ntiles), dim3(kSThreads), 0, s, x, y, z,
+                               (const uint32_t*)nullptr, (const float4*)nullptr, (uint32_t)n, op, shift,
                                (const uint32_t*)hist, w.keys[1].as<uint32_t>(), pdst);
             kcur = 1;
         } else {
-            hipLaunchKernelGGL((k_scatter<false, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
-                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), psrc, (uint32_t)n, op, shift, ntiles,
+            hipLaunchKernelGGL((k_scatter<false, KeyOp>), dim3(ntiles), dim3(kSThreads), 0, s, x, y, z,
+                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), psrc, (uint32_t)n, op, shift,
                                (const uint32_t*)hist, w.keys[kdst].as<uint32_t>(), pdst);
             kcur = kdst;
         }
