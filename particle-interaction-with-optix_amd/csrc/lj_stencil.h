@@ -161,7 +161,9 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
                 const float gy = axis_gap(pi.y, yy, cy, g.lo[1], g.cw[1], tr.margin);
                 const float w2 = tr.r2 - gz * gz - gy * gy;
                 if (w2 < 0.f) continue;
-                const float w = __builtin_sqrtf(w2) + tr.margin;
+                // hardware square root (about 1 ulp, no correctly rounded fix-up):
+                // its error is far inside the margin, which exceeds rc 2^-14
+                const float w = __builtin_amdgcn_sqrtf(w2) + tr.margin;
                 xa = max(x0, trim_cell(pi.x - w, g.lo[0], tr.inv_cw, g.gmax[0]));
                 xb = min(x1, trim_cell(pi.x + w, g.lo[0], tr.inv_cw, g.gmax[0]));
             }
