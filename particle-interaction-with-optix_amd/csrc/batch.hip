@@ -116,6 +116,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, true> acc;
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
+    acc.allow_exact_diff(c, pi, active);
     const uint32_t np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, Rx, R, tr, c,
                                                         queue + threadIdx.x, blockDim.x, depth, acc);
     if (!active) return;

@@ -37,6 +37,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ 
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR> acc;
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
+    acc.allow_exact_diff(c, pi, active);
     const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u,
                                                               Rx, R, tr, c, queue + threadIdx.x, blockDim.x, depth, acc);
     double esum = 0.0, vsum = 0.0;

@@ -309,6 +309,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
     if (k < n) {
         const float4 pi = leaf[k];
         acc.set_i(pi);
+        acc.allow_exact_diff(c, pi, true);  // over the lanes with a query
         const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, list, blockDim.x, [&](int, float4 pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float d2 = dist2(dx, dy, dz);

@@ -67,14 +67,36 @@ template <int PREC, bool SOFT, bool VIR>
 struct PairSums {
     Acc<PREC> ax, ay, az, ae, av;
     double px = 0.0, py = 0.0, pz = 0.0;  // p_i in FP64 (MIXED)
+    bool exact = false;                   // wave-uniform, see allow_exact_diff
     __device__ inline void set_i(float4 pi) {
         px = pi.x;
         py = pi.y;
         pz = pi.z;
     }
+    // Accepted partners of a cutoff predicate lie within rc of p_i on every axis.  If
+    // |p_i| >= 2 rc on every axis (so p_j has the sign of p_i and lies in [p_i/2, 2 p_i]),
+    // Sterbenz's lemma makes the FP32 difference p_j - p_i exact, and converting it
+    // gives the FP64 difference bit for bit with one instruction per axis instead of
+    // two.  The fast form is used only when it holds for every calling lane, so the
+    // choice is a uniform branch: call it from every lane of the wave (idle lanes with
+    // active = false), or from the lanes of a branch that alone uses the flag.
+    __device__ inline void allow_exact_diff(const PairConst& c, float4 pi, bool active) {
+        const float reach = 2.001f * c.rc;
+        const bool ok = !active || (!SOFT && fabsf(pi.x) >= reach && fabsf(pi.y) >= reach && fabsf(pi.z) >= reach);
+        exact = __all(ok);
+    }
     __device__ inline void add(const PairConst& c, float4 pj, float dx, float dy, float dz, float d2) {
         if constexpr (PREC == LJ_PREC_MIXED) {
-            const double ddx = (double)pj.x - px, ddy = (double)pj.y - py, ddz = (double)pj.z - pz;
+            double ddx, ddy, ddz;
+            if (exact) {
+                ddx = dx;
+                ddy = dy;
+                ddz = dz;
+            } else {
+                ddx = (double)pj.x - px;
+                ddy = (double)pj.y - py;
+                ddz = (double)pj.z - pz;
+            }
             double r2 = __builtin_fma(ddz, ddz, __builtin_fma(ddy, ddy, ddx * ddx));
             if constexpr (SOFT) r2 += c.softd;
             const double ir2 = rcp_f64(r2);

@@ -32,6 +32,7 @@ __global__ __launch_bounds__(256) void k_list_force(const float* __restrict__ x,
     if (i < n) {
         const float4 pi = make_float4(x[i], y[i], z[i], 0.f);
         acc.set_i(pi);
+        acc.allow_exact_diff(c, pi, true);  // over the lanes with a particle
         const unsigned long long r0 = rows[i], r1 = rows[i + 1];
         for (unsigned long long q = r0; q < r1; ++q) {
             const uint32_t j = cols[q];
