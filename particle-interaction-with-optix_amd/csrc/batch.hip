@@ -26,9 +26,8 @@ struct SysDesc {
     uint32_t first, n, cell_base, ccell_base;
 };
 
-// Largest s with base[s] <= v (base ascending, base[0] = 0).
-__device__ inline uint32_t find_sys(const uint32_t* __restrict__ base, uint32_t nsys, uint32_t v) {
-    uint32_t lo = 0, hi = nsys;  // answer in [lo, hi)
+// Largest s in [lo, hi) with base[s] <= v (base ascending, base[lo] <= v).
+__device__ inline uint32_t find_sys_in(const uint32_t* __restrict__ base, uint32_t lo, uint32_t hi, uint32_t v) {
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (base[mid] <= v) lo = mid;
@@ -37,14 +36,39 @@ __device__ inline uint32_t find_sys(const uint32_t* __restrict__ base, uint32_t 
     return lo;
 }
 
+// Search hints: hint[j] = the system of value j << kHintShift.  The system of v is
+// monotone in v, so it lies between hint[v >> kHintShift] and the next entry: a search
+// over one or two systems instead of all of them.  Table of (total >> kHintShift) + 2
+// entries for values below `total`.
+constexpr int kHintShift = 10;
+
+__host__ __device__ inline size_t hint_entries(uint64_t total) { return (size_t)(total >> kHintShift) + 2; }
+
+__global__ __launch_bounds__(256) void k_build_hints(const uint32_t* __restrict__ base, uint32_t nsys, uint32_t m,
+                                                     uint32_t* __restrict__ hint) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint64_t v = (uint64_t)j << kHintShift;
+    hint[j] = find_sys_in(base, 0, nsys, v > 0xffffffffull ? 0xffffffffu : (uint32_t)v);
+}
+
+// Largest s with base[s] <= v, v below the table's total.
+__device__ inline uint32_t find_sys(const uint32_t* __restrict__ base, const uint32_t* __restrict__ hint, uint32_t nsys,
+                                    uint32_t v) {
+    const uint32_t j = v >> kHintShift;
+    const uint32_t lo = hint[j], up = hint[j + 1] + 1;
+    return find_sys_in(base, lo, up < nsys ? up : nsys, v);
+}
+
 // Global cell key of particle i: its system's first cell plus the cell in that
 // system's grid.
 struct BatchKeyOp {
     const SysDesc* sys;
     const uint32_t* first;
+    const uint32_t* hint;
     uint32_t nsys;
     __device__ uint32_t operator()(float x, float y, float z, uint32_t i) const {
-        const SysDesc& d = sys[find_sys(first, nsys, i)];
+        const SysDesc& d = sys[find_sys(first, hint, nsys, i)];
         return d.cell_base + cell_key(d.g, x, y, z);
     }
 };
@@ -52,41 +76,55 @@ struct BatchKeyOp {
 // Particle counts of the cell_edge grids from the force grids' offsets (refine > 1).
 __global__ __launch_bounds__(256) void k_batch_coarse_counts(const uint32_t* __restrict__ foff, uint32_t nccells,
                                                              const SysDesc* __restrict__ sys,
-                                                             const uint32_t* __restrict__ ccbase, uint32_t nsys, int Kx,
+                                                             const uint32_t* __restrict__ ccbase,
+                                                             const uint32_t* __restrict__ cchint, uint32_t nsys, int Kx,
                                                              int K, uint32_t* __restrict__ ccount) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nccells) return;
-    const SysDesc& d = sys[find_sys(ccbase, nsys, c)];
+    const SysDesc& d = sys[find_sys(ccbase, cchint, nsys, c)];
     ccount[c] = coarse_count(foff + d.cell_base, d.g, d.cg, Kx, K, c - d.ccell_base);
 }
 
 // 27-cell candidates per system on the cell_edge grids (offsets coff over the
 // concatenated cell_edge cells), ComputeNbInteractions semantics (cudaLJ.cu:255-297).
+// A wave's cells usually belong to one system: it then adds its sum with one atomic.
 __global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __restrict__ coff, uint32_t nccells,
                                                           const SysDesc* __restrict__ sys,
-                                                          const uint32_t* __restrict__ ccbase, uint32_t nsys,
+                                                          const uint32_t* __restrict__ ccbase,
+                                                          const uint32_t* __restrict__ cchint, uint32_t nsys,
                                                           lj_result* __restrict__ res) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nccells) return;
-    const uint32_t nh = coff[c + 1] - coff[c];
-    if (!nh) return;
-    const uint32_t s = find_sys(ccbase, nsys, c);
-    const GridGeom& g = sys[s].cg;
-    const uint32_t base = sys[s].ccell_base;
-    int32_t cx, cy, cz;
-    cell_coords(g, c - base, cx, cy, cz);
-    const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
-    const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
-    unsigned long long sum = 0;
-    for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
-        if (zz < 0 || zz >= gz) continue;
-        for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
-            if (yy < 0 || yy >= gy) continue;
-            const uint32_t row = base + (uint32_t)((zz * gy + yy) * gx);
-            sum += coff[row + x1 + 1] - coff[row + x0];
+    uint32_t s = 0xffffffffu;
+    unsigned long long v = 0;
+    if (c < nccells) {
+        s = find_sys(ccbase, cchint, nsys, c);
+        const uint32_t nh = coff[c + 1] - coff[c];
+        if (nh) {
+            const GridGeom& g = sys[s].cg;
+            const uint32_t base = sys[s].ccell_base;
+            int32_t cx, cy, cz;
+            cell_coords(g, c - base, cx, cy, cz);
+            const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
+            const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
+            unsigned long long sum = 0;
+            for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
+                if (zz < 0 || zz >= gz) continue;
+                for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
+                    if (yy < 0 || yy >= gy) continue;
+                    const uint32_t row = base + (uint32_t)((zz * gy + yy) * gx);
+                    sum += coff[row + x1 + 1] - coff[row + x0];
+                }
+            }
+            v = (unsigned long long)nh * (sum - 1);
         }
     }
-    atomicAdd(&res[s].n_candidates, (unsigned long long)nh * (sum - 1));
+    const uint32_t s0 = __shfl(s, 0, kWaveSize);
+    if (__all(s == s0 || s == 0xffffffffu)) {
+        v = wave_sum(v);
+        if (__lane_id() == 0 && v) atomicAdd(&res[s0].n_candidates, v);
+    } else if (v) {
+        atomicAdd(&res[s].n_candidates, v);
+    }
 }
 
 // One thread per sorted slot, as the single-system kernel (the same stencil loop and
@@ -99,7 +137,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
                                                            const uint32_t* __restrict__ cell_of,
                                                            const uint32_t* __restrict__ off, uint32_t n,
                                                            const SysDesc* __restrict__ sys,
-                                                           const uint32_t* __restrict__ cbase, uint32_t nsys, int Rx,
+                                                           const uint32_t* __restrict__ cbase,
+                                                           const uint32_t* __restrict__ chint, uint32_t nsys, int Rx,
                                                            int R,
                                                            uint32_t depth, float* __restrict__ fx, float* __restrict__ fy,
                                                            float* __restrict__ fz, float* __restrict__ e,
@@ -108,7 +147,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
     const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = k < n;
     const uint32_t key = active ? cell_of[k] : 0u;
-    const SysDesc& d = sys[find_sys(cbase, nsys, key)];
+    const SysDesc& d = sys[find_sys(cbase, chint, nsys, key)];
     const GridGeom g = d.g;
     const PairConst c = d.c;
     const Trim tr = d.tr;
@@ -136,6 +175,9 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const SysDesc* __restrict_
     const SysDesc& d = sys[blockIdx.x];
     double a = 0.0, b = 0.0;
     unsigned long long np = 0;
+    // unrolled so that the loads of several iterations are in flight; each thread still
+    // adds its values in index order
+#pragma unroll 8
     for (uint32_t q = threadIdx.x; q < d.n; q += 256) {
         const double2 v = ev[d.first + q];
         a += v.x;
@@ -222,17 +264,38 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     std::memcpy(blob.data() + dbytes, first.data(), ubytes);
     std::memcpy(blob.data() + dbytes + ubytes, cbase.data(), ubytes);
     std::memcpy(blob.data() + dbytes + 2 * ubytes, ccbase.data(), ubytes);
+    const SysDesc* dsys = nullptr;
+    const uint32_t *dfirst = nullptr, *dcbase = nullptr, *dccbase = nullptr;
+    // search hints of the three base arrays (particles, force-grid cells, cell_edge cells)
+    const size_t h1 = hint_entries(ntot), h2 = hint_entries(ctot), h3 = hint_entries(cctot);
+    auto bind = [&]() {
+        dsys = W.sys.as<SysDesc>();
+        const uint8_t* dplan = W.sys.as<uint8_t>();
+        dfirst = reinterpret_cast<const uint32_t*>(dplan + dbytes);
+        dcbase = reinterpret_cast<const uint32_t*>(dplan + dbytes + ubytes);
+        dccbase = reinterpret_cast<const uint32_t*>(dplan + dbytes + 2 * ubytes);
+    };
     if (blob != W.plan) {
         LJ_HIP(ctx, W.sys.grow(blob.size()));
+        LJ_HIP(ctx, W.hints.grow((h1 + h2 + h3) * sizeof(uint32_t)));
         LJ_HIP(ctx, hipMemcpyAsync(W.sys.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+        bind();
+        uint32_t* hw = W.hints.as<uint32_t>();
+        const uint32_t* bases[3] = {dfirst, dcbase, dccbase};
+        const size_t sizes[3] = {h1, h2, h3};
+        for (int t = 0; t < 3; ++t) {
+            hipLaunchKernelGGL(k_build_hints, dim3(div_up((int64_t)sizes[t], 256)), dim3(256), 0, s, bases[t],
+                               (uint32_t)nsys, (uint32_t)sizes[t], hw);
+            hw += sizes[t];
+        }
+        LJ_HIP(ctx, hipGetLastError());
         LJ_HIP(ctx, hipStreamSynchronize(s));
         W.plan = std::move(blob);
     }
-    const SysDesc* dsys = W.sys.as<SysDesc>();
-    const uint8_t* dplan = W.sys.as<uint8_t>();
-    const uint32_t* dfirst = reinterpret_cast<const uint32_t*>(dplan + dbytes);
-    const uint32_t* dcbase = reinterpret_cast<const uint32_t*>(dplan + dbytes + ubytes);
-    const uint32_t* dccbase = reinterpret_cast<const uint32_t*>(dplan + dbytes + 2 * ubytes);
+    bind();
+    const uint32_t* hfirst = W.hints.as<uint32_t>();
+    const uint32_t* hcbase = hfirst + h1;
+    const uint32_t* hccbase = hcbase + h2;
 
     const int64_t n = (int64_t)ntot;
     const uint32_t ncells = (uint32_t)ctot, nccells = (uint32_t)cctot;
@@ -245,7 +308,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         fill_u64(reinterpret_cast<unsigned long long*>(results), (uint32_t)(4 * nsys), 0ull, s);
         int bits = 0;
         while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)ncells) ++bits;
-        LJ_TRY(sort_particles(ctx, x, y, z, n, BatchKeyOp{dsys, dfirst, (uint32_t)nsys}, bits, s));
+        LJ_TRY(sort_particles(ctx, x, y, z, n, BatchKeyOp{dsys, dfirst, hfirst, (uint32_t)nsys}, bits, s));
         LJ_TRY(cell_offsets_from_sorted(ctx, n, ncells, s));
         off = B.offsets.as<uint32_t>();
     }
@@ -256,14 +319,14 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
             LJ_HIP(ctx, B.ccounts.grow((size_t)nccells * 4));
             LJ_HIP(ctx, B.coffsets.grow(((size_t)nccells + 1) * 4));
             hipLaunchKernelGGL(k_batch_coarse_counts, dim3(div_up(nccells, 256)), dim3(256), 0, s, (const uint32_t*)off,
-                               nccells, dsys, dccbase, (uint32_t)nsys, sh.refine_x, sh.refine,
+                               nccells, dsys, dccbase, hccbase, (uint32_t)nsys, sh.refine_x, sh.refine,
                                B.ccounts.as<uint32_t>());
             uint32_t* co = B.coffsets.as<uint32_t>();
             LJ_TRY(scan_u32(ctx, B.ccounts.as<uint32_t>(), co, nccells, co + nccells, s));
             coff = co;
         }
         hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(nccells, 256)), dim3(256), 0, s, coff, nccells, dsys,
-                           dccbase, (uint32_t)nsys, results);
+                           dccbase, hccbase, (uint32_t)nsys, results);
     }
     double2* ev = W.ev.as<double2>();
     uint32_t* npair = W.npair.as<uint32_t>();
@@ -276,7 +339,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         if (n > 0) {
 #define LJ_BFQ(PRED, PREC, Q)                                                                                     \
     hipLaunchKernelGGL((k_batch_force<PRED, PREC, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of,            \
-                       (const uint32_t*)off, (uint32_t)n, dsys, dcbase, (uint32_t)nsys, sh.refine_x, sh.refine, \
+                       (const uint32_t*)off, (uint32_t)n, dsys, dcbase, hcbase, (uint32_t)nsys, sh.refine_x, sh.refine, \
                        (uint32_t)sh.depth, \
                        fx, fy, fz, e, ev, npair)
 #define LJ_BF(PRED, PREC)                          \

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI* in, TS* ou
 // previous run's key up to its own key, and the last slot writes n into the cells above
 // the last key.  Every cell is written exactly once, so no fill, per-cell counts or
 // scan are needed.  A gap of more than kGapInline cells is cut into pieces of at most
-// kGapPiece cells, appended to a list that k_fill_gaps spreads over whole workgroups
+// kGapPiece cells, appended to a list that k_fill_gaps spreads over whole waves
 // (sparse grids: one lane never loops over a long empty stretch).
 constexpr uint32_t kGapInline = 256;
 constexpr uint32_t kGapPiece = 4096;
@@ -119,14 +119,21 @@ struct GapPiece {
     uint32_t a, b, v;  // off[a .. b) = v
 };
 
+// off[a .. b) = v for each lane's range: short ranges by the lane itself, longer ones
+// cut into pieces appended to the list (one atomic per wave: lanes take consecutive
+// entries from a wave-wide prefix sum).  Every lane of the wave calls this.
 __device__ inline void fill_cells(uint32_t a, uint32_t b, uint32_t v, uint32_t* __restrict__ off,
                                   GapPiece* __restrict__ gaps, uint32_t* __restrict__ ngaps) {
-    if (b - a <= kGapInline) {
+    const uint32_t len = b - a;
+    const bool piecewise = len > kGapInline;
+    if (!piecewise)
         for (uint32_t c = a; c < b; ++c) off[c] = v;
-        return;
-    }
-    const uint32_t pieces = (b - a + kGapPiece - 1) / kGapPiece;
-    const uint32_t at = atomicAdd(ngaps, pieces);
+    if (!__any(piecewise)) return;
+    const uint32_t pieces = piecewise ? (len + kGapPiece - 1) / kGapPiece : 0u;
+    const uint32_t incl = wave_incl_scan_t(pieces);
+    uint32_t at = 0;
+    if (lane_id() == kWave - 1) at = atomicAdd(ngaps, incl);
+    at = __shfl(at, kWave - 1, kWave) + incl - pieces;
     for (uint32_t i = 0; i < pieces; ++i) {
         const uint32_t pa = a + i * kGapPiece;
         gaps[at + i] = GapPiece{pa, b - pa > kGapPiece ? pa + kGapPiece : b, v};
@@ -137,22 +144,37 @@ __global__ __launch_bounds__(256) void k_offsets(const uint32_t* __restrict__ sk
                                                  uint32_t* __restrict__ off, GapPiece* __restrict__ gaps,
                                                  uint32_t* __restrict__ ngaps) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t key = skeys[k];
-    if (k == 0) fill_cells(0u, key + 1u, 0u, off, gaps, ngaps);
-    else {
-        const uint32_t prev = skeys[k - 1];
-        if (prev != key) fill_cells(prev + 1u, key + 1u, k, off, gaps, ngaps);
+    uint32_t a = 0, b = 0, v = 0, key = 0;  // the cells (previous key, key] take slot k
+    if (k < n) {
+        key = skeys[k];
+        if (k == 0) {
+            b = key + 1u;
+        } else {
+            const uint32_t prev = skeys[k - 1];
+            if (prev != key) {
+                a = prev + 1u;
+                b = key + 1u;
+                v = k;
+            }
+        }
     }
-    if (k == n - 1) fill_cells(key + 1u, ncells + 1u, n, off, gaps, ngaps);
+    fill_cells(a, b, v, off, gaps, ngaps);
+    // the cells above the last key take n (the wave holding slot n - 1)
+    if (__any(k == n - 1)) {
+        const bool last = k == n - 1;
+        fill_cells(last ? key + 1u : 0u, last ? ncells + 1u : 0u, n, off, gaps, ngaps);
+    }
 }
 
+// One wave per listed piece.
 __global__ __launch_bounds__(256) void k_fill_gaps(const GapPiece* __restrict__ gaps,
                                                    const uint32_t* __restrict__ ngaps, uint32_t* __restrict__ off) {
     const uint32_t m = *ngaps;
-    for (uint32_t e = blockIdx.x; e < m; e += gridDim.x) {
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave, nwaves = gridDim.x * blockDim.x / kWave;
+    for (uint32_t e = wave; e < m; e += nwaves) {
         const GapPiece p = gaps[e];
-        for (uint32_t c = p.a + threadIdx.x; c < p.b; c += blockDim.x) off[c] = p.v;
+        for (uint32_t c = p.a + lane; c < p.b; c += kWave) off[c] = p.v;
     }
 }
 

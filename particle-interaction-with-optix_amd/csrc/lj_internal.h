@@ -79,6 +79,7 @@ struct LbvhWork {
 
 struct BatchWork {
     DevBuf sys;           // device copy of the plan: per-system descriptors and cell bases
+    DevBuf hints;         // search hints of the plan's three base arrays (rebuilt with the plan)
     DevBuf npair;         // u32 accepted-pair count of each particle (caller order)
     DevBuf ev;            // double2 energy and virial of each particle (caller order)
     std::vector<uint8_t> plan;  // host copy of the plan last uploaded to sys
