@@ -53,6 +53,10 @@ static int grid_of(lj_context* ctx, const lj_params* p, GridGeom* g) {
 }
 
 int make_geom(lj_context* ctx, const lj_params* p, int64_t n, GridGeom* g, PairConst* c) {
+    // every byte defined, padding included: the batched path compares plans bytewise to
+    // decide whether to upload them again
+    std::memset(g, 0, sizeof *g);
+    std::memset(c, 0, sizeof *c);
     if (!p) return fail(ctx, LJ_ERR_INVALID_ARG, "params is NULL");
     if (n < 0 || n > 0x7fffffffll) return fail(ctx, LJ_ERR_INVALID_ARG, "particle count %lld out of range [0, 2^31-1]", (long long)n);
     if (!(p->rc > 0.f) || !std::isfinite(p->rc)) return fail(ctx, LJ_ERR_INVALID_ARG, "rc must be finite and > 0");

@@ -49,6 +49,9 @@ struct Status {
 };
 
 // Workspace of one binned system (or of a concatenated batch).
+// Entries past the last sorted particle in BinWork::pos (never written, read but unused).
+constexpr int kPosSpare = 2;
+
 struct BinWork {
     DevBuf keys[2];      // u32 radix keys (ping-pong)
     DevBuf hist;         // radix digit histograms

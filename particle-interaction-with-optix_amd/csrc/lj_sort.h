@@ -279,7 +279,8 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ x, const
 
 // Sorts n particles by op's key (the low `bits` bits).  The sorted keys land in
 // ctx->bin.keys[ctx->bin.cur] and the payload (x, y, z, bits of the original index)
-// in ctx->bin.pos, which has room for n + 1 entries (entry n is not written).
+// in ctx->bin.pos, which has room for n + kPosSpare entries (the spare ones are not
+// written).
 template <class KeyOp>
 int sort_particles(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const KeyOp& op,
                    int bits, hipStream_t s) {
@@ -288,10 +289,10 @@ int sort_particles(lj_context* ctx, const float* x, const float* y, const float*
     const size_t nn = (size_t)(n > 0 ? n : 1);
     LJ_HIP(ctx, w.keys[0].grow(nn * 4));
     LJ_HIP(ctx, w.keys[1].grow(nn * 4));
-    // one slot beyond the last particle: the force loops read pairs of slots (t, t+1)
-    // without clamping t+1 and use the second only when it is in range
-    LJ_HIP(ctx, w.pos.grow((nn + 1) * sizeof(float4)));
-    LJ_HIP(ctx, w.pos_alt.grow((nn + 1) * sizeof(float4)));
+    // spare slots beyond the last particle: the force loops read groups of slots
+    // (t .. t + kCandidates - 1) without clamping and use a slot only when it is in range
+    LJ_HIP(ctx, w.pos.grow((nn + kPosSpare) * sizeof(float4)));
+    LJ_HIP(ctx, w.pos_alt.grow((nn + kPosSpare) * sizeof(float4)));
     w.cur = 0;
     if (n <= 0) return LJ_OK;
     const int passes = (n > 1) ? (bits + kBits - 1) / kBits : 0;

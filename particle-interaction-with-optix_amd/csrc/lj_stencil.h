@@ -43,6 +43,10 @@ struct Shape {
 };
 
 constexpr int kMaxBlock = 256;
+// candidates loaded and tested per iteration of the stencil's row loop (the sorted
+// positions carry kCandidates - 1 spare entries, so the loads need no clamp)
+constexpr int kCandidates = 3;
+static_assert(kCandidates - 1 <= kPosSpare, "unclamped candidate loads need spare slots");
 constexpr int kMaxDepth = 32;
 
 __device__ inline void cell_coords(const GridGeom& g, uint32_t key, int32_t& cx, int32_t& cy, int32_t& cz) {
@@ -169,14 +173,18 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
             }
             const uint32_t row = cbase + (uint32_t)((zz * g.g[1] + yy) * g.g[0]);
             const uint32_t b = off[row + xa], en = off[row + xb + 1];
-            for (uint32_t t = b; t < en; t += 2) {
+            for (uint32_t t = b; t < en; t += kCandidates) {
                 if constexpr (QUEUE)
-                    if (__any(pc + 2 > depth)) flush();
-                // slot t + 1 <= n exists (pos has a spare entry); it counts only below en
-                const float4 p0 = pos[t];
-                const float4 p1 = pos[t + 1];
-                visit(t, p0);
-                if (t + 1 < en) visit(t + 1, p1);
+                    if (__any(pc + kCandidates > depth)) flush();
+                // slots up to t + kCandidates - 1 <= n + kCandidates - 2 exist (pos has
+                // spare entries); they count only below en
+                float4 pj[kCandidates];
+#pragma unroll
+                for (int u = 0; u < kCandidates; ++u) pj[u] = pos[t + u];
+                visit(t, pj[0]);
+#pragma unroll
+                for (int u = 1; u < kCandidates; ++u)
+                    if (t + u < en) visit(t + u, pj[u]);
             }
         }
     }
