@@ -1,8 +1,9 @@
 """Parity at the benchmark sizes through properties that do not need an FP64 run over
 every particle: bit-exact binning against the oracle at 16.7M particles, the closed-form
 candidate count, Newton's third law and the energy total, LBVH = cell grid on the 1M
-sphere and the 4.2M clusters, and every one of the 4,096 batched systems equal to its
-own call.  (Force errors against FP64 at these sizes: test_c3_precision_subsample.)
+sphere and the 4.2M clusters, every one of the 4,096 batched systems equal to its own
+call, and binning plus subsampled forces at 2^27 particles.  (Force errors against FP64
+at the benchmark sizes: test_c3_precision_subsample.)
 """
 import numpy as np
 import pytest
@@ -119,3 +120,32 @@ def test_c4_batched_equals_single_calls_full(engine):
     np.testing.assert_allclose(rb["e_total"], rs["e_total"], rtol=1e-12, atol=0)
     for k in ("fx", "fy", "fz", "e"):
         assert torch.equal(out[k][:n].view(torch.int32), single[k].view(torch.int32)), k
+
+
+def test_large_n_bin_and_forces(engine):
+    """2^27 = 134,217,728 uniform particles, rc = 0.005 (8,000,000 cells, ~9.4e9 pairs):
+    eight times the largest benchmark input.  Binning bit-exact and the closed-form
+    candidate count against the oracle, forces of a 4,096-particle subsample against
+    FP64."""
+    n = 1 << 27
+    x, y, z = ljamd.generate("uniform", 4242, n)
+    p = ljamd.make_params(0.005)
+    q = po.from_lj(p)
+    print(f"[large] n={n}", flush=True)
+    off_ref, perm_ref, _ = po.bin(q, x, y, z)
+    dx, dy, dz = engine.to_device(x, y, z)
+    off, perm, cand = engine.bin(dx, dy, dz, p)
+    engine.torch.cuda.synchronize()
+    assert np.array_equal(off.cpu().numpy().view(np.uint32), off_ref)
+    assert np.array_equal(perm[:n].cpu().numpy().view(np.uint32), perm_ref)
+    assert int(cand.cpu()[0]) == po.candidates(q, off_ref)
+    del off, perm
+    (fx, fy, fz, e), res = _forces(engine, dx, dy, dz, p)
+    sub = np.arange(0, n, n // 4096)
+    ref = po.forces_f64(q, x, y, z, subset=sub, nthreads=16)
+    fin = (np.abs(ref["fx"]) < 3e38) & (np.abs(ref["fy"]) < 3e38) & (np.abs(ref["fz"]) < 3e38)
+    err = po.rel_errors([a[sub][fin] for a in (fx, fy, fz)], (ref["fx"][fin], ref["fy"][fin], ref["fz"][fin]))["rel"]
+    print(f"[large] pairs={int(res['n_pairs'])} max rel {err.max():.3e}", flush=True)
+    assert err.max() <= 1e-5
+    assert int(res["n_pairs"]) % 2 == 0
+    assert int(res["n_pairs"]) == pytest.approx(4 / 3 * np.pi * 0.005**3 * n * n, rel=0.02)
