@@ -236,6 +236,8 @@ __device__ inline bool overlaps(float4 p, float h, float4 lo, float4 hi) {
 // 16 entries and 64-thread blocks measured fastest (list 8..64, blocks 64..256).
 constexpr int kList = 16;
 constexpr int kLbvhBlock = 64;
+constexpr int kLeafGroup = 2;  // bucket leaves loaded and tested per step of the scan
+static_assert(kLeafGroup - 1 <= kPosSpare, "unclamped leaf loads need spare entries");
 
 template <class F>
 __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __restrict__ range,
@@ -243,8 +245,7 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
                                 float4 p, float h, unsigned long long& tests, int* __restrict__ list, uint32_t stride,
                                 F&& f) {
     if (n == 1) return true;
-    auto leaf_test = [&](int slot) {
-        const float4 pj = leaf[slot];
+    auto leaf_test = [&](int slot, const float4& pj) {
         if ((pj.x - p.x) <= h && (p.x - pj.x) <= h && (pj.y - p.y) <= h && (p.y - pj.y) <= h &&
             (pj.z - p.z) <= h && (p.z - pj.z) <= h) {
             ++tests;
@@ -277,14 +278,23 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
                 node = overlaps(p, h, lo, hi) ? __float_as_int(lo.w) : __float_as_int(hi.w);
             }
         }
-        // phase 2
+        // phase 2: a bucket's leaves in groups of kLeafGroup, loaded together (the leaf
+        // array has spare entries past the last leaf, so the loads need no clamp)
         for (unsigned q = 0; q < cnt; ++q) {
             const int e = list[q * stride];
             if (e >= n - 1) {
-                leaf_test(e - (n - 1));
+                leaf_test(e - (n - 1), leaf[e - (n - 1)]);
             } else {
                 const int2 r = range[e];
-                for (int slot = r.x; slot <= r.y; ++slot) leaf_test(slot);
+                for (int slot = r.x; slot <= r.y; slot += kLeafGroup) {
+                    float4 pj[kLeafGroup];
+#pragma unroll
+                    for (int u = 0; u < kLeafGroup; ++u) pj[u] = leaf[slot + u];
+                    leaf_test(slot, pj[0]);
+#pragma unroll
+                    for (int u = 1; u < kLeafGroup; ++u)
+                        if (slot + u <= r.y) leaf_test(slot + u, pj[u]);
+                }
             }
         }
     }
