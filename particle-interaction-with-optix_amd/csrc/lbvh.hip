@@ -121,9 +121,15 @@ __global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, uint32_t 
     if (i < n) p[i] = v;
 }
 
-constexpr int kChunkBits = 4;  // a chunk of level l + 1 holds 16 chunks of level l
+constexpr int kChunkBits = 2;  // a chunk of level l + 1 holds 4 chunks of level l
 constexpr int kChunk = 1 << kChunkBits;
-constexpr int kLevels = 6;     // chunks of 16, 256, 4096, 65536, 2^20 and 2^24 leaves
+constexpr int kLevels = 24 / kChunkBits;  // chunks of 4, 16, ..., 2^24 leaves
+static_assert(kLevels * kChunkBits == 24, "the largest chunk holds 2^24 leaves");
+
+// First chunk pair of each level in the chunk array (level l holds ceil(n / 4^(l+1))).
+struct ChunkLevels {
+    uint32_t at[kLevels];
+};
 
 struct Bounds {
     float lo[3], hi[3];
@@ -167,14 +173,14 @@ __global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__
 }
 
 // Bounds of internal node i: min/max of the leaf positions in its range, from the
-// largest aligned chunks inside the range and single leaves at its ends (at most 15
-// elements per level on each side).  Level l (0 .. kLevels-1, chunks of
-// 16^(l + 1) leaves) starts at chunks + 2 * l * stride.  The .w lanes carry the walk:
-// lo.w = the left child, hi.w = the rope (references as int bits).
+// largest aligned chunks inside the range and single leaves at its ends (at most
+// kChunk - 1 elements per level on each side; narrow chunks keep that short).  Level l
+// (0 .. kLevels-1, chunks of kChunk^(l + 1) leaves) starts at pair lv.at[l] of chunks.
+// The .w lanes carry the walk: lo.w = the left child, hi.w = the rope (int bits).
 __global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ leaf, int n,
                                                      const int2* __restrict__ range, const int2* __restrict__ child,
                                                      const int* __restrict__ rope_by_end,
-                                                     const float4* __restrict__ chunks, uint32_t stride,
+                                                     const float4* __restrict__ chunks, ChunkLevels lv,
                                                      float4* __restrict__ box) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -189,7 +195,7 @@ __global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ 
             const float4 p = leaf[e];
             bb.add(p, p);
         } else {
-            const float4* c = chunks + 2 * ((size_t)l * stride + e);
+            const float4* c = chunks + 2 * ((size_t)lv.at[l] + e);
             bb.add(c[0], c[1]);
         }
     };
@@ -411,9 +417,18 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     LJ_HIP(ctx, L.rope.grow(nn * sizeof(int)));
     LJ_HIP(ctx, L.flags.grow(64));
     LJ_HIP(ctx, L.node_box.grow(nn * 2 * sizeof(float4)));
-    // chunk bounds: kLevels levels of ceil(n / 16) pairs each (the higher levels use
-    // only their first entries)
-    LJ_HIP(ctx, L.chunks.grow((size_t)kLevels * div_up((int64_t)nn, kChunk) * 2 * sizeof(float4)));
+    // chunk bounds: level l holds ceil(n / kChunk^(l + 1)) (lo, hi) pairs, levels back to back
+    ChunkLevels lv;
+    uint64_t pairs = 0;
+    {
+        uint64_t cnt = (uint64_t)nn;
+        for (int l = 0; l < kLevels; ++l) {
+            cnt = div_up(cnt, kChunk);
+            lv.at[l] = (uint32_t)pairs;
+            pairs += cnt;
+        }
+    }
+    LJ_HIP(ctx, L.chunks.grow((size_t)pairs * 2 * sizeof(float4)));
     MortonKeyOp op;
     for (int d = 0; d < 3; ++d) {
         op.lo[d] = p->box_lo[d];
@@ -430,16 +445,14 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
         hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, s, code, (int)n, L.node_child.as<int2>(),
                            range, rope);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, overflow_flag(ctx, n), 1u, 0);
-    // chunk bounds, level by level (level 0 from the leaves, level l + 1 from level l);
-    // level l holds ceil(n / 16^(l + 1)) pairs at chunks + 2 * l * stride
+    // chunk bounds, level by level (level 0 from the leaves, level l + 1 from level l)
     const float4* leaves = B.pos.as<float4>();
-    const uint32_t stride = (uint32_t)div_up(n, kChunk);
     {
         uint64_t cnt = (uint64_t)n;
         const float4* in = leaves;
         for (int l = 0; l < kLevels; ++l) {
             const uint64_t nch = div_up(cnt, kChunk);
-            float4* out = L.chunks.as<float4>() + 2 * (size_t)l * stride;
+            float4* out = L.chunks.as<float4>() + 2 * (size_t)lv.at[l];
             hipLaunchKernelGGL(k_chunk_bounds, dim3((unsigned)div_up(cnt, 256)), dim3(256), 0, s, in, l == 0,
                                (uint32_t)cnt, (uint32_t)nch, out);
             in = out;
@@ -449,7 +462,7 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     if (n > 1)
         hipLaunchKernelGGL(k_node_bounds, dim3(div_up(n - 1, 256)), dim3(256), 0, s, leaves, (int)n,
                            (const int2*)range, (const int2*)L.node_child.as<int2>(), (const int*)rope,
-                           (const float4*)L.chunks.as<float4>(), stride, L.node_box.as<float4>());
+                           (const float4*)L.chunks.as<float4>(), lv, L.node_box.as<float4>());
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }

@@ -75,7 +75,7 @@ struct LbvhWork {
     DevBuf node_box;      // float4 x2 per internal node (lo, hi)
     DevBuf node_child;    // int2 per internal node
     DevBuf range;         // int2 per internal node: first and last leaf it covers
-    DevBuf chunks;        // float4 x2 bounds of aligned leaf chunks (256, 65536, 2^24 leaves)
+    DevBuf chunks;        // float4 x2 bounds of aligned leaf chunks (levels of 4, 16, ..., 2^24 leaves)
     DevBuf rope;          // int per leaf: the node after every subtree ending at that leaf
     DevBuf flags;         // the traversal overflow flag
 };
