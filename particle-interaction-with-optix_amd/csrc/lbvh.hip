@@ -77,9 +77,10 @@ __device__ inline int delta(const uint32_t* __restrict__ code, int n, int i, int
 // Node numbering: internal nodes 0..n-2 (0 is the root), leaf k is node n-1+k (n <=
 // 2^30, so every id fits 31 bits).  A child or rope reference with bit 31 set is a
 // bucket: an internal node covering at most kBucket leaves, scanned instead of walked.
-// 16 measured fastest over 4..32 (4,194,304 clustered particles, rc = 0.005: 8 leaves
-// 5.1 ms per step, 16 leaves 4.6 ms, with kList 16 and 128-thread blocks).
-constexpr int kBucket = 16;
+// Measured on 4,194,304 clustered particles, rc = 0.005 (per step): 8 leaves 5.1 ms, 16
+// leaves 4.6 ms with one leaf scanned at a time; with leaves scanned in pairs 12: 4.34,
+// 16: 4.20, 24: 4.15, 32: 4.16 ms.
+constexpr int kBucket = 24;
 constexpr int kCollapsed = (int)0x80000000u;
 constexpr int kSentinel = 0x7fffffff;  // the rope of the last subtree: end of the walk
 
