@@ -1,12 +1,13 @@
 // lj_sort.h -- stable LSD radix sort of particles by a 32-bit key, carrying the payload
 // (x, y, z, original index) through every pass.
 //
-// One pass over 8 key bits is:
-//   histogram  per 2048-key tile, tile-major table hist[tile * 256 + d] (one
-//              contiguous 1 KB row per tile)
+// Digits are 8 bits, or 9 bits for keys of 25 to 27 bits (three passes instead of
+// four).  One pass over RB key bits (R = 2^RB digits) is:
+//   histogram  per 2048-key tile, tile-major table hist[tile * R + d] (one contiguous
+//              row per tile)
 //   offsets    a digit-major exclusive scan of that table in three small kernels:
-//              hist[tile * 256 + d] becomes the first output slot of digit d's keys
-//              in that tile
+//              hist[tile * R + d] becomes the first output slot of digit d's keys in
+//              that tile
 //   scatter    each wave ranks its own contiguous run of keys stably with per-wave LDS
 //              counters (no block barrier per round), the waves' counts are combined
 //              once, the tile is staged in LDS in digit order, and every digit's run
@@ -24,8 +25,6 @@ namespace lj {
 namespace sortk {
 
 constexpr int kWave = 64;
-constexpr int kBits = 8;
-constexpr int kRadix = 1 << kBits;
 constexpr int kThreads = 256;  // histogram
 constexpr int kItems = 8;
 constexpr int kTile = kThreads * kItems;  // 2048 keys per tile
@@ -35,7 +34,7 @@ constexpr int kSThreads = 1024;
 constexpr int kSItems = kTile / kSThreads;
 constexpr int kSWaves = kSThreads / kWave;
 constexpr int kWaveKeys = kSItems * kWave;  // the contiguous keys of one wave in a tile
-static_assert(kSThreads >= kRadix && kSItems * kSThreads == kTile, "scatter shape");
+static_assert(kSThreads >= 512 && kSItems * kSThreads == kTile, "scatter shape");
 constexpr uint32_t kMaxChunks = 256;       // row groups of the offset scan
 
 __device__ inline unsigned rank_below(uint64_t mask) {
@@ -57,12 +56,13 @@ __device__ inline void load_elem(uint32_t idx, const float* __restrict__ x, cons
     }
 }
 
-template <bool FIRST, class KeyOp>
+template <int RB, bool FIRST, class KeyOp>
 __global__ __launch_bounds__(kThreads) void k_hist(const float* __restrict__ x, const float* __restrict__ y,
                                                    const float* __restrict__ z, const uint32_t* __restrict__ kin,
                                                    uint32_t n, KeyOp op, int shift, uint32_t* __restrict__ hist) {
+    constexpr int kRadix = 1 << RB;
     __shared__ uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
+    for (int d = threadIdx.x; d < kRadix; d += kThreads) h[d] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kTile;
 #pragma unroll
@@ -76,14 +76,16 @@ __global__ __launch_bounds__(kThreads) void k_hist(const float* __restrict__ x, 
         }
     }
     __syncthreads();
-    hist[(uint64_t)blockIdx.x * kRadix + threadIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < kRadix; d += kThreads) hist[(uint64_t)blockIdx.x * kRadix + d] = h[d];
 }
 
 // ---- digit-major exclusive scan of the tile-major table, rows grouped in chunks of
 // `chunk` consecutive tiles (at most kMaxChunks chunks).
-// part[c * 256 + d] = sum of digit d over the rows of chunk c.
-static __global__ __launch_bounds__(kRadix) void k_dig_reduce(const uint32_t* __restrict__ hist, uint32_t ntiles,
-                                                       uint32_t chunk, uint32_t* __restrict__ part) {
+// part[c * R + d] = sum of digit d over the rows of chunk c (one thread per digit).
+template <int RB>
+__global__ __launch_bounds__(1 << RB) void k_dig_reduce(const uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                        uint32_t chunk, uint32_t* __restrict__ part) {
+    constexpr int kRadix = 1 << RB;
     const uint32_t t0 = blockIdx.x * chunk;
     const uint32_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
     uint32_t s = 0;
@@ -92,15 +94,17 @@ static __global__ __launch_bounds__(kRadix) void k_dig_reduce(const uint32_t* __
     part[blockIdx.x * kRadix + threadIdx.x] = s;
 }
 
-// One block of 4 x 256 threads; thread (q, d) owns a quarter of the chunks of digit d.
-// part[c * 256 + d] becomes the first output slot of digit d's keys in chunk c:
-// (keys of smaller digits) + (keys of digit d in earlier chunks).
-static __global__ __launch_bounds__(4 * kRadix) void k_dig_scan(uint32_t* __restrict__ part, uint32_t nchunks) {
-    __shared__ uint32_t qsum[4][kRadix];
+// One block of 1024 threads = kQ groups of R; thread (q, d) owns a 1/kQ share of the
+// chunks of digit d.  part[c * R + d] becomes the first output slot of digit d's keys in
+// chunk c: (keys of smaller digits) + (keys of digit d in earlier chunks).
+template <int RB>
+__global__ __launch_bounds__(1024) void k_dig_scan(uint32_t* __restrict__ part, uint32_t nchunks) {
+    constexpr int kRadix = 1 << RB, kQ = 1024 / kRadix;
+    __shared__ uint32_t qsum[kQ][kRadix];
     __shared__ uint32_t dbase[kRadix];
     __shared__ uint32_t wsum[kRadix / kWave];
     const unsigned d = threadIdx.x % kRadix, q = threadIdx.x / kRadix;
-    const uint32_t per = (nchunks + 3) / 4;
+    const uint32_t per = (nchunks + kQ - 1) / kQ;
     const uint32_t c0 = q * per < nchunks ? q * per : nchunks;
     const uint32_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
     uint32_t s = 0;
@@ -109,8 +113,9 @@ static __global__ __launch_bounds__(4 * kRadix) void k_dig_scan(uint32_t* __rest
     qsum[q][d] = s;
     __syncthreads();
     uint32_t tot = 0, incl = 0;
-    if (q == 0) {  // threads 0..255 (waves 0-3): inclusive scan of the digit totals
-        tot = qsum[0][d] + qsum[1][d] + qsum[2][d] + qsum[3][d];
+    if (q == 0) {  // threads 0..R-1: inclusive scan of the digit totals
+#pragma unroll
+        for (int k = 0; k < kQ; ++k) tot += qsum[k][d];
         incl = tot;
 #pragma unroll
         for (int k = 1; k < kWave; k <<= 1) {
@@ -136,9 +141,11 @@ static __global__ __launch_bounds__(4 * kRadix) void k_dig_scan(uint32_t* __rest
     }
 }
 
-// hist[t * 256 + d] <- first output slot of digit d's keys in tile t.
-static __global__ __launch_bounds__(kRadix) void k_dig_down(uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t chunk,
-                                                     const uint32_t* __restrict__ part) {
+// hist[t * R + d] <- first output slot of digit d's keys in tile t.
+template <int RB>
+__global__ __launch_bounds__(1 << RB) void k_dig_down(uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t chunk,
+                                                      const uint32_t* __restrict__ part) {
+    constexpr int kRadix = 1 << RB;
     const uint32_t t0 = blockIdx.x * chunk;
     const uint32_t t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
     uint32_t run = part[blockIdx.x * kRadix + threadIdx.x];
@@ -155,12 +162,13 @@ static __global__ __launch_bounds__(kRadix) void k_dig_down(uint32_t* __restrict
 // share an L2) sorts its keys stably by digit and writes each digit's run.  Element j
 // of the tile (input order) is handled by wave j / kWaveKeys, round
 // (j % kWaveKeys) / 64, lane j % 64, so (wave, round, lane) is the input order.
-template <bool FIRST, class KeyOp>
+template <int RB, bool FIRST, class KeyOp>
 __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__ x, const float* __restrict__ y,
                                                        const float* __restrict__ z, const uint32_t* __restrict__ kin,
                                                        const float4* __restrict__ pin, uint32_t n, KeyOp op, int shift,
                                                        const uint32_t* __restrict__ hist, uint32_t* __restrict__ kout,
                                                        float4* __restrict__ pout) {
+    constexpr int kRadix = 1 << RB;
     __shared__ uint32_t skey[kTile];
     __shared__ float4 spay[kTile];
     __shared__ uint32_t wcnt[kSWaves][kRadix];  // per-wave digit counts, then per-wave first slots
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
         const uint32_t d = (key[r] >> shift) & (kRadix - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < kBits; ++b) {
+        for (int b = 0; b < RB; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bb = __ballot(bit);
             peers &= bit ? bb : ~bb;
@@ -207,7 +215,7 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
         rank[r] = before + rk;
     }
     __syncthreads();
-    // threads 0..255, one per digit: staging start of the digit (exclusive scan over
+    // threads 0..R-1, one per digit: staging start of the digit (exclusive scan over
     // digits of the tile's counts) and of each wave's part of it
     uint32_t c[kSWaves];
     uint32_t tot = 0, incl = 0;
@@ -277,6 +285,61 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ x, const
 
 }  // namespace sortk
 
+namespace sortk {
+
+// The radix passes of a sort with RB-bit digits (passes >= 1, n >= 2).
+template <int RB, class KeyOp>
+int sort_passes(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const KeyOp& op,
+                int passes, hipStream_t s) {
+    constexpr int kRadix = 1 << RB;
+    BinWork& w = ctx->bin;
+    const unsigned ntiles = div_up(n, kTile);
+    const uint32_t chunk = div_up(ntiles, kMaxChunks);
+    const uint32_t nchunks = div_up(ntiles, chunk);
+    // the table, then the chunk partials
+    LJ_HIP(ctx, w.hist.grow(((size_t)ntiles + kMaxChunks) * kRadix * 4));
+    uint32_t* hist = w.hist.as<uint32_t>();
+    uint32_t* part = hist + (size_t)ntiles * kRadix;
+    // the last pass must write ctx->bin.pos
+    float4* pbuf[2] = {w.pos.as<float4>(), w.pos_alt.as<float4>()};
+    int kcur = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * RB;
+        const int kdst = p == 0 ? 0 : kcur ^ 1;
+        float4* pdst = pbuf[(passes - 1 - p) & 1];
+        const float4* psrc = pbuf[(passes - p) & 1];
+        if (p == 0) {
+            hipLaunchKernelGGL((k_hist<RB, true, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
+                               (const uint32_t*)nullptr, (uint32_t)n, op, shift, hist);
+        } else {
+            hipLaunchKernelGGL((k_hist<RB, false, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
+                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), (uint32_t)n, op, shift, hist);
+        }
+        hipLaunchKernelGGL(k_dig_reduce<RB>, dim3(nchunks), dim3(kRadix), 0, s, (const uint32_t*)hist, ntiles, chunk,
+                           part);
+        hipLaunchKernelGGL(k_dig_scan<RB>, dim3(1), dim3(1024), 0, s, part, nchunks);
+        hipLaunchKernelGGL(k_dig_down<RB>, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, chunk,
+                           (const uint32_t*)part);
+        if (p == 0) {
+            // pass 0 reads positions; it writes keys[1] so that keys[0] stays free
+            hipLaunchKernelGGL((k_scatter<RB, true, KeyOp>), dim3(ntiles), dim3(kSThreads), 0, s, x, y, z,
+                               (const uint32_t*)nullptr, (const float4*)nullptr, (uint32_t)n, op, shift,
+                               (const uint32_t*)hist, w.keys[1].as<uint32_t>(), pdst);
+            kcur = 1;
+        } else {
+            hipLaunchKernelGGL((k_scatter<RB, false, KeyOp>), dim3(ntiles), dim3(kSThreads), 0, s, x, y, z,
+                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), psrc, (uint32_t)n, op, shift,
+                               (const uint32_t*)hist, w.keys[kdst].as<uint32_t>(), pdst);
+            kcur = kdst;
+        }
+    }
+    w.cur = kcur;
+    LJ_HIP(ctx, hipGetLastError());
+    return LJ_OK;
+}
+
+}  // namespace sortk
+
 // Sorts n particles by op's key (the low `bits` bits).  The sorted keys land in
 // ctx->bin.keys[ctx->bin.cur] and the payload (x, y, z, bits of the original index)
 // in ctx->bin.pos, which has room for n + kPosSpare entries (the spare ones are not
@@ -295,56 +358,15 @@ int sort_particles(lj_context* ctx, const float* x, const float* y, const float*
     LJ_HIP(ctx, w.pos_alt.grow((nn + kPosSpare) * sizeof(float4)));
     w.cur = 0;
     if (n <= 0) return LJ_OK;
-    const int passes = (n > 1) ? (bits + kBits - 1) / kBits : 0;
-    if (passes == 0) {
+    if (n == 1 || bits == 0) {
         hipLaunchKernelGGL((k_pack<KeyOp>), dim3(div_up(n, 256)), dim3(256), 0, s, x, y, z, (uint32_t)n, op,
                            w.keys[0].as<uint32_t>(), w.pos.as<float4>());
         LJ_HIP(ctx, hipGetLastError());
         return LJ_OK;
     }
-    const unsigned ntiles = div_up(n, kTile);
-    const uint32_t chunk = div_up(ntiles, kMaxChunks);
-    const uint32_t nchunks = div_up(ntiles, chunk);
-    // the table, then the chunk partials
-    LJ_HIP(ctx, w.hist.grow(((size_t)ntiles + kMaxChunks) * kRadix * 4));
-    uint32_t* hist = w.hist.as<uint32_t>();
-    uint32_t* part = hist + (size_t)ntiles * kRadix;
-    // the last pass must write ctx->bin.pos
-    float4* pbuf[2] = {w.pos.as<float4>(), w.pos_alt.as<float4>()};
-    int kcur = 0;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = p * kBits;
-        const int kdst = p == 0 ? 0 : kcur ^ 1;
-        float4* pdst = pbuf[(passes - 1 - p) & 1];
-        const float4* psrc = pbuf[(passes - p) & 1];
-        if (p == 0) {
-            hipLaunchKernelGGL((k_hist<true, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
-                               (const uint32_t*)nullptr, (uint32_t)n, op, shift, hist);
-        } else {
-            hipLaunchKernelGGL((k_hist<false, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
-                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), (uint32_t)n, op, shift, hist);
-        }
-        hipLaunchKernelGGL(k_dig_reduce, dim3(nchunks), dim3(kRadix), 0, s, (const uint32_t*)hist, ntiles, chunk,
-                           part);
-        hipLaunchKernelGGL(k_dig_scan, dim3(1), dim3(4 * kRadix), 0, s, part, nchunks);
-        hipLaunchKernelGGL(k_dig_down, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, chunk,
-                           (const uint32_t*)part);
-        if (p == 0) {
-            // pass 0 reads positions; it writes keys[1] so that keys[0] stays free
-            hipLaunchKernelGGL((k_scatter<true, KeyOp>), dim3(ntiles), dim3(kSThreads), 0, s, x, y, z,
-                               (const uint32_t*)nullptr, (const float4*)nullptr, (uint32_t)n, op, shift,
-                               (const uint32_t*)hist, w.keys[1].as<uint32_t>(), pdst);
-            kcur = 1;
-        } else {
-            hipLaunchKernelGGL((k_scatter<false, KeyOp>), dim3(ntiles), dim3(kSThreads), 0, s, x, y, z,
-                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), psrc, (uint32_t)n, op, shift,
-                               (const uint32_t*)hist, w.keys[kdst].as<uint32_t>(), pdst);
-            kcur = kdst;
-        }
-    }
-    w.cur = kcur;
-    LJ_HIP(ctx, hipGetLastError());
-    return LJ_OK;
+    // 25- to 27-bit keys: three passes of 9 bits instead of four of 8
+    if (bits > 24 && bits <= 27) return sort_passes<9>(ctx, x, y, z, n, op, 3, s);
+    return sort_passes<8>(ctx, x, y, z, n, op, (bits + 7) / 8, s);
 }
 
 }  // namespace lj

@@ -45,10 +45,15 @@ def _case(name):
     if name == "one_cell_crowd":
         x, y, z = ljamd.generate("uniform", 11, 3000, [0.30, 0.31])
         return (x, y, z), ljamd.make_params(0.05)
+    if name == "sparse_26bit_keys":
+        # 333^3 = 36,926,037 cells: a 26-bit key, sorted in three 9-bit passes
+        x, y, z = ljamd.generate("uniform", 12, 200_000)
+        return (x, y, z), ljamd.make_params(0.003)
     raise KeyError(name)
 
 
-BIN_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7", "single", "empty", "edges", "one_cell_crowd"]
+BIN_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7", "single", "empty", "edges", "one_cell_crowd",
+             "sparse_26bit_keys"]
 
 
 @pytest.mark.parametrize("name", BIN_CASES)

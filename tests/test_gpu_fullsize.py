@@ -95,11 +95,13 @@ def test_lbvh_equals_cellgrid_full(engine, name):
 
 def test_c4_batched_equals_single_calls_full(engine):
     """All 4,096 systems of the batched workload (46.8M particles): one batched launch
-    gives every system the bits and counts of its own call with the same shape."""
+    gives every system the bits and counts of its own call with the same shape.  With
+    x-cells of cell_edge / 4 the batch spans 64M cells, a 26-bit key sorted in three
+    9-bit passes (each single call sorts its own small key space in 8-bit passes)."""
     torch = engine.torch
     systems, X, Y, Z = workloads.batch_c4()
     for sy in systems:
-        sy.params.refine, sy.params.refine_x, sy.params.block, sy.params.depth = 1, 1, 256, 1
+        sy.params.refine, sy.params.refine_x, sy.params.block, sy.params.depth = 1, 4, 256, 1
     n = X.size
     print(f"[full batched] c4 systems={len(systems)} n={n}", flush=True)
     dx, dy, dz = engine.to_device(X, Y, Z)
