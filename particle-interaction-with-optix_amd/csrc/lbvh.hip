@@ -8,7 +8,7 @@
 // 1e-16 <= d2 <= c^2 and adds the LJ energy (optixLJ.cu:134-176).
 //
 // Here:
-//  1. 30-bit Morton codes of the positions in the params box, stable radix sort
+//  1. 27-bit Morton codes of the positions in the params box, stable radix sort
 //     (ties keep index order, so the sorted (code, index) keys are unique);
 //  2. the hierarchy of Karras, "Maximizing Parallelism in the Construction of BVHs,
 //     Octrees, and k-d Trees", HPG 2012 (one thread per internal node);
@@ -50,18 +50,23 @@ __device__ inline uint32_t spread10(uint32_t v) {
     return v;
 }
 
-__device__ inline uint32_t quant10(float x, float lo, float inv) {
+// Morton precision: 2^kMortonBits steps per axis of the box.  The codes only order the
+// leaves (node bounds come from the positions, equal codes are split by index), and
+// 27-bit codes sort in three 9-bit radix passes where 30-bit ones take four.
+constexpr int kMortonBits = 9;
+
+__device__ inline uint32_t quant(float x, float lo, float inv) {
     float q = (x - lo) * inv;
-    q = fminf(fmaxf(q, 0.0f), 1023.0f);
+    q = fminf(fmaxf(q, 0.0f), (float)((1 << kMortonBits) - 1));
     return (uint32_t)q;
 }
 
-// 30-bit Morton code of a position, quantised to 1024 steps per axis of the box.
+// 3 kMortonBits-bit Morton code of a position.
 struct MortonKeyOp {
     float lo[3], inv[3];
     __device__ uint32_t operator()(float x, float y, float z, uint32_t) const {
-        return (spread10(quant10(x, lo[0], inv[0])) << 2) | (spread10(quant10(y, lo[1], inv[1])) << 1) |
-               spread10(quant10(z, lo[2], inv[2]));
+        return (spread10(quant(x, lo[0], inv[0])) << 2) | (spread10(quant(y, lo[1], inv[1])) << 1) |
+               spread10(quant(z, lo[2], inv[2]));
     }
 };
 
@@ -433,10 +438,10 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     MortonKeyOp op;
     for (int d = 0; d < 3; ++d) {
         op.lo[d] = p->box_lo[d];
-        op.inv[d] = 1024.0f / (p->box_hi[d] - p->box_lo[d]);
+        op.inv[d] = (float)(1 << kMortonBits) / (p->box_hi[d] - p->box_lo[d]);
     }
     // the sorted payload is the leaf array: positions in Morton order, w = original index
-    LJ_TRY(sort_particles(ctx, x, y, z, n, op, 30, s));
+    LJ_TRY(sort_particles(ctx, x, y, z, n, op, 3 * kMortonBits, s));
     if (n == 0) return LJ_OK;
     const uint32_t* code = B.sorted_keys();
     int2* range = L.range.as<int2>();
