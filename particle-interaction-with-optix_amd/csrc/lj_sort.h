@@ -34,7 +34,7 @@ constexpr int kSThreads = 1024;
 constexpr int kSItems = kTile / kSThreads;
 constexpr int kSWaves = kSThreads / kWave;
 constexpr int kWaveKeys = kSItems * kWave;  // the contiguous keys of one wave in a tile
-static_assert(kSThreads >= 512 && kSItems * kSThreads == kTile, "scatter shape");
+static_assert(kSItems * kSThreads == kTile, "scatter shape");
 constexpr uint32_t kMaxChunks = 256;       // row groups of the offset scan
 
 __device__ inline unsigned rank_below(uint64_t mask) {
@@ -100,6 +100,7 @@ __global__ __launch_bounds__(1 << RB) void k_dig_reduce(const uint32_t* __restri
 template <int RB>
 __global__ __launch_bounds__(1024) void k_dig_scan(uint32_t* __restrict__ part, uint32_t nchunks) {
     constexpr int kRadix = 1 << RB, kQ = 1024 / kRadix;
+    static_assert(kQ >= 1 && kQ * kRadix == 1024, "digit count must divide the block");
     __shared__ uint32_t qsum[kQ][kRadix];
     __shared__ uint32_t dbase[kRadix];
     __shared__ uint32_t wsum[kRadix / kWave];
@@ -169,6 +170,7 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
                                                        const uint32_t* __restrict__ hist, uint32_t* __restrict__ kout,
                                                        float4* __restrict__ pout) {
     constexpr int kRadix = 1 << RB;
+    static_assert(kRadix <= kSThreads, "one thread per digit combines the waves' counts");
     __shared__ uint32_t skey[kTile];
     __shared__ float4 spay[kTile];
     __shared__ uint32_t wcnt[kSWaves][kRadix];  // per-wave digit counts, then per-wave first slots
