@@ -191,31 +191,34 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* p, uint32_
 }
 
 // sum over cells of n_c * (S_c - 1), S_c = the clipped 27-cell count including c;
-// equals the per-particle count of ComputeNbInteractions (cudaLJ.cu:278-292).
+// equals the per-particle count of ComputeNbInteractions (cudaLJ.cu:278-292).  A fixed
+// grid strides over the cells and adds one sum per workgroup: on sparse grids (millions
+// of mostly empty cells) one atomic per 256 cells to a single counter was most of the
+// kernel's time.
+constexpr unsigned kCandBlocks = 1024;
+
 __global__ __launch_bounds__(256) void k_candidates(const uint32_t* __restrict__ off, GridGeom g,
                                                     unsigned long long* __restrict__ out) {
     __shared__ unsigned long long wsum[256 / kWave];
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
     unsigned long long v = 0;
-    if (c < g.ncells) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < g.ncells; c += gridDim.x * blockDim.x) {
         const uint32_t nh = off[c + 1] - off[c];
-        if (nh) {
-            const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
-            const int32_t cx = (int32_t)(c % (uint32_t)gx);
-            const int32_t cy = (int32_t)((c / (uint32_t)gx) % (uint32_t)gy);
-            const int32_t cz = (int32_t)(c / ((uint32_t)gx * (uint32_t)gy));
-            const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
-            unsigned long long s = 0;
-            for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
-                if (zz < 0 || zz >= gz) continue;
-                for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
-                    if (yy < 0 || yy >= gy) continue;
-                    const uint32_t row = (uint32_t)((zz * gy + yy) * gx);
-                    s += off[row + x1 + 1] - off[row + x0];
-                }
+        if (!nh) continue;
+        const int32_t cx = (int32_t)(c % (uint32_t)gx);
+        const int32_t cy = (int32_t)((c / (uint32_t)gx) % (uint32_t)gy);
+        const int32_t cz = (int32_t)(c / ((uint32_t)gx * (uint32_t)gy));
+        const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
+        unsigned long long sum = 0;
+        for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
+            if (zz < 0 || zz >= gz) continue;
+            for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
+                if (yy < 0 || yy >= gy) continue;
+                const uint32_t row = (uint32_t)((zz * gy + yy) * gx);
+                sum += off[row + x1 + 1] - off[row + x0];
             }
-            v = (unsigned long long)nh * (s - 1);
         }
+        v += (unsigned long long)nh * (sum - 1);
     }
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
@@ -335,7 +338,8 @@ int extract_perm(lj_context* ctx, int64_t n, uint32_t* perm, hipStream_t s) {
 int count_candidates_from(lj_context* ctx, const GridGeom& g, const uint32_t* offsets, unsigned long long* out,
                           hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, out, 1u, 0ull);
-    hipLaunchKernelGGL(k_candidates, dim3(div_up(g.ncells, 256)), dim3(256), 0, s, offsets, g, out);
+    const unsigned nb = div_up(g.ncells > 0 ? g.ncells : 1, 256);
+    hipLaunchKernelGGL(k_candidates, dim3(nb < kCandBlocks ? nb : kCandBlocks), dim3(256), 0, s, offsets, g, out);
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }
