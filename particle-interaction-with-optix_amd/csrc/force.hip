@@ -274,8 +274,9 @@ Trim make_trim(const lj_params* p, const GridGeom& g) {
         mag = std::fmax(mag, std::fabs((double)p->box_hi[d]));
         t.hi[d] = p->box_hi[d];
     }
-    // 2^-19 mag bounds the rounding of x_i +- w, of v - lo and (divided by the cell
-    // edge) the <= 3 ulp gap between (v - lo) * inv_cw and (v - lo) / cw
+    // 2^-19 mag bounds the rounding of v - lo and (times the cell edge) the few ulp
+    // by which (v - lo) inv_cw -+ w inv_cw and the cell coordinate (v - lo) / cw differ
+    // from their exact values; the rc term covers the square root of the window
     const double cwmax = std::fmax((double)g.cw[0], std::fmax((double)g.cw[1], (double)g.cw[2]));
     t.margin = (float)(0x1p-19 * mag + (double)p->rc * 0x1p-14 + 8.0 * 0x1p-24 * cwmax);
     t.inv_cw = 1.0f / g.cw[0];

@@ -14,11 +14,12 @@ namespace lj {
 
 // Stencil trimming constants: a row of the stencil is skipped when its (y,z) distance
 // to particle i exceeds the cutoff, and the x-cells of a row are limited to the cells
-// of [x_i - w, x_i + w] with w = sqrt(rc^2 - gap^2), found by multiplying with the
-// rounded reciprocal of the cell edge.  Every bound is widened by `margin`, which
-// exceeds the FP32 rounding of x_i +- w and the difference between that product and
-// the cell assignment's correctly rounded quotient (a few ulp of the cell coordinate),
-// so a row or cell is dropped only when no particle in it can pass the predicate.
+// of [x_i - w, x_i + w] with w = sqrt(rc^2 - gap^2), found in cell units as
+// (x_i - lo) inv_cw -+ w inv_cw with the rounded reciprocal inv_cw of the cell edge.
+// Every bound is widened by `margin`, which exceeds the FP32 rounding of these steps
+// and their difference from the cell assignment's correctly rounded quotient (a few
+// ulp of the cell coordinate), so a row or cell is dropped only when no particle in
+// it can pass the predicate.
 struct Trim {
     float r2;       // (rc (1 + 2^-16))^2, rounded up
     float margin;   // absolute widening of every bound
@@ -26,11 +27,6 @@ struct Trim {
     float inv_cw;   // 1 / x cell edge of the force grid, rounded
     int on;         // 0 for the CUDALJ mode (no cutoff)
 };
-
-// Cell of coordinate v by the reciprocal, clamped like axis_cell (trimming only).
-__device__ inline int32_t trim_cell(float v, float lo, float inv, float gmax) {
-    return (int32_t)fminf(fmaxf((v - lo) * inv, 0.0f), gmax);
-}
 
 // Kernel shape of a force launch: force-grid refinement (cells of edge cell_edge/Kx
 // along x and cell_edge/K along y and z; the stencil reaches Kx cells along x and K
@@ -157,6 +153,10 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
     const int32_t z0 = active ? (cz - R > 0 ? cz - R : 0) : 1;
     const int32_t z1 = active ? (cz + R < g.g[2] - 1 ? cz + R : g.g[2] - 1) : 0;
     const int32_t y0 = cy - R > 0 ? cy - R : 0, y1 = cy + R < g.g[1] - 1 ? cy + R : g.g[1] - 1;
+    // x cell coordinate of particle i by the reciprocal (trimming only): the x-cells of
+    // a row are those of qx -+ w inv_cw, truncated (a negative lower bound truncates to
+    // 0, like the clamp of the cell rule) and clipped to [x0, x1]
+    const float qx = (pi.x - g.lo[0]) * tr.inv_cw;
     for (int32_t zz = z0; zz <= z1; ++zz) {
         const float gz = trim ? axis_gap(pi.z, zz, cz, g.lo[2], g.cw[2], tr.margin) : 0.f;
         for (int32_t yy = y0; yy <= y1; ++yy) {
@@ -168,23 +168,29 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
                 // hardware square root (about 1 ulp, no correctly rounded fix-up):
                 // its error is far inside the margin, which exceeds rc 2^-14
                 const float w = __builtin_amdgcn_sqrtf(w2) + tr.margin;
-                xa = max(x0, trim_cell(pi.x - w, g.lo[0], tr.inv_cw, g.gmax[0]));
-                xb = min(x1, trim_cell(pi.x + w, g.lo[0], tr.inv_cw, g.gmax[0]));
+                xa = min(max(x0, (int32_t)__builtin_fmaf(-w, tr.inv_cw, qx)), x1);
+                xb = min(x1, (int32_t)__builtin_fmaf(w, tr.inv_cw, qx));
             }
             const uint32_t row = cbase + (uint32_t)((zz * g.g[1] + yy) * g.g[0]);
             const uint32_t b = off[row + xa], en = off[row + xb + 1];
+            // candidate t + u exists iff t + u < en, i.e. t < enu[u] = max(en - u, 0)
+            uint32_t enu[kCandidates];
+#pragma unroll
+            for (int u = 1; u < kCandidates; ++u) enu[u] = en > (uint32_t)u ? en - (uint32_t)u : 0u;
             for (uint32_t t = b; t < en; t += kCandidates) {
                 if constexpr (QUEUE)
                     if (__any(pc + kCandidates > depth)) flush();
                 // slots up to t + kCandidates - 1 <= n + kCandidates - 2 exist (pos has
-                // spare entries); they count only below en
+                // spare entries); they count only below en.  One 64-bit base, the
+                // candidates at immediate offsets from it.
+                const float4* pt = pos + t;
                 float4 pj[kCandidates];
 #pragma unroll
-                for (int u = 0; u < kCandidates; ++u) pj[u] = pos[t + u];
+                for (int u = 0; u < kCandidates; ++u) pj[u] = pt[u];
                 visit(t, pj[0]);
 #pragma unroll
                 for (int u = 1; u < kCandidates; ++u)
-                    if (t + u < en) visit(t + u, pj[u]);
+                    if (t < enu[u]) visit(t + u, pj[u]);
             }
         }
     }
