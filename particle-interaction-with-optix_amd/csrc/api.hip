@@ -183,7 +183,7 @@ static void release_all(lj_context* ctx) {
         }
     BinWork& b = ctx->bin;
     for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.hist, &b.scan_tmp, &b.gaps, &b.offsets, &b.pos, &b.pos_alt,
-                      &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets})
+                      &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets, &b.cand})
         d->release();
     LbvhWork& l = ctx->lbvh;
     for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags})

@@ -108,9 +108,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI* in, TS* ou
 // keys: the first slot k of each run of equal keys writes k into the cells above the
 // previous run's key up to its own key, and the last slot writes n into the cells above
 // the last key.  Every cell is written exactly once, so no fill, per-cell counts or
-// scan are needed.  A gap of more than kGapInline cells is cut into pieces of at most
-// kGapPiece cells, appended to a list that k_fill_gaps spreads over whole waves
-// (sparse grids: one lane never loops over a long empty stretch).
+// scan are needed.  A lane writes a gap of up to kGapLane cells itself; its wave writes
+// a gap of up to kGapInline cells, 64 cells per store; a longer gap is cut into pieces
+// of at most kGapPiece cells, appended to a list that k_fill_gaps spreads over whole
+// waves (sparse grids: one lane never loops over a long empty stretch).
+constexpr uint32_t kGapLane = 8;
 constexpr uint32_t kGapInline = 256;
 constexpr uint32_t kGapPiece = 4096;
 constexpr unsigned kGapBlocks = 1024;
@@ -119,15 +121,22 @@ struct GapPiece {
     uint32_t a, b, v;  // off[a .. b) = v
 };
 
-// off[a .. b) = v for each lane's range: short ranges by the lane itself, longer ones
-// cut into pieces appended to the list (one atomic per wave: lanes take consecutive
-// entries from a wave-wide prefix sum).  Every lane of the wave calls this.
+// off[a .. b) = v for each lane's range: short ranges by the lane itself, medium ones
+// by the whole wave one range at a time, longer ones cut into pieces appended to the
+// list (one atomic per wave: lanes take consecutive entries from a wave-wide prefix
+// sum).  Every lane of the wave calls this.
 __device__ inline void fill_cells(uint32_t a, uint32_t b, uint32_t v, uint32_t* __restrict__ off,
                                   GapPiece* __restrict__ gaps, uint32_t* __restrict__ ngaps) {
     const uint32_t len = b - a;
     const bool piecewise = len > kGapInline;
-    if (!piecewise)
+    if (len <= kGapLane)
         for (uint32_t c = a; c < b; ++c) off[c] = v;
+    for (uint64_t m = __ballot(len > kGapLane && !piecewise); m; m &= m - 1) {
+        const int l = __builtin_ctzll(m);
+        const uint32_t la = __builtin_amdgcn_readlane(a, l), lb = __builtin_amdgcn_readlane(b, l);
+        const uint32_t lv = __builtin_amdgcn_readlane(v, l);
+        for (uint32_t c = la + lane_id(); c < lb; c += kWave) off[c] = lv;
+    }
     if (!__any(piecewise)) return;
     const uint32_t pieces = piecewise ? (len + kGapPiece - 1) / kGapPiece : 0u;
     const uint32_t incl = wave_incl_scan_t(pieces);
@@ -191,43 +200,78 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* p, uint32_
 }
 
 // sum over cells of n_c * (S_c - 1), S_c = the clipped 27-cell count including c;
-// equals the per-particle count of ComputeNbInteractions (cudaLJ.cu:278-292).  A fixed
-// grid strides over the cells and adds one sum per workgroup: on sparse grids (millions
-// of mostly empty cells) one atomic per 256 cells to a single counter was most of the
-// kernel's time.
-constexpr unsigned kCandBlocks = 1024;
+// equals the per-particle count of ComputeNbInteractions (cudaLJ.cu:278-292).  One
+// wave per (y,z) row of cells at a time, lanes along x: the row's neighbour rows are
+// wave-uniform, so a cell's 27-cell count is nine x-range differences of the offsets
+// (coalesced across the lanes) and no per-cell index arithmetic.  Each workgroup
+// writes its sum to part[blockIdx.x] and k_sum_parts adds them up: on sparse grids
+// (millions of mostly empty cells) the walk is a chain of dependent loads, so it needs
+// many waves in flight, and atomics of that many workgroups to one counter would
+// serialise.
+constexpr unsigned kCandBlocks = 8192;
 
 __global__ __launch_bounds__(256) void k_candidates(const uint32_t* __restrict__ off, GridGeom g,
-                                                    unsigned long long* __restrict__ out) {
-    __shared__ unsigned long long wsum[256 / kWave];
+                                                    unsigned long long* __restrict__ part) {
+    constexpr unsigned kWaves = 256 / kWave;
+    __shared__ unsigned long long wsum[kWaves];
     const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
+    const uint32_t nrows = (uint32_t)gy * (uint32_t)gz;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int32_t lane = (int32_t)lane_id();
     unsigned long long v = 0;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < g.ncells; c += gridDim.x * blockDim.x) {
-        const uint32_t nh = off[c + 1] - off[c];
-        if (!nh) continue;
-        const int32_t cx = (int32_t)(c % (uint32_t)gx);
-        const int32_t cy = (int32_t)((c / (uint32_t)gx) % (uint32_t)gy);
-        const int32_t cz = (int32_t)(c / ((uint32_t)gx * (uint32_t)gy));
-        const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
-        unsigned long long sum = 0;
-        for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
-            if (zz < 0 || zz >= gz) continue;
-            for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
-                if (yy < 0 || yy >= gy) continue;
-                const uint32_t row = (uint32_t)((zz * gy + yy) * gx);
-                sum += off[row + x1 + 1] - off[row + x0];
-            }
+    for (uint32_t r = blockIdx.x * kWaves + wave; r < nrows; r += gridDim.x * kWaves) {
+        const int32_t cy = (int32_t)(r % (uint32_t)gy), cz = (int32_t)(r / (uint32_t)gy);
+        const int32_t z0 = cz > 0 ? cz - 1 : 0, z1 = cz < gz - 1 ? cz + 1 : gz - 1;
+        const int32_t y0 = cy > 0 ? cy - 1 : 0, y1 = cy < gy - 1 ? cy + 1 : gy - 1;
+        const uint32_t* home = off + (size_t)r * (size_t)gx;
+        for (int32_t cx = lane; cx < gx; cx += kWave) {
+            const uint32_t nh = home[cx + 1] - home[cx];
+            if (!nh) continue;
+            // x-cells x0 .. xe - 1 of each neighbour row (the counts of at most n
+            // particles fit 32 bits)
+            const int32_t x0 = cx > 0 ? cx - 1 : 0, xe = cx < gx - 1 ? cx + 2 : gx;
+            uint32_t sum = 0;
+            for (int32_t zz = z0; zz <= z1; ++zz)
+                for (int32_t yy = y0; yy <= y1; ++yy) {
+                    const uint32_t* rw = off + ((size_t)zz * (size_t)gy + (size_t)yy) * (size_t)gx;
+                    sum += rw[xe] - rw[x0];
+                }
+            v += (unsigned long long)nh * (sum - 1);
         }
-        v += (unsigned long long)nh * (sum - 1);
     }
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
-    if (lane_id() == 0) wsum[threadIdx.x / kWave] = v;
+    if (lane == 0) wsum[wave] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (int w = 0; w < 256 / kWave; ++w) t += wsum[w];
-        if (t) atomicAdd(out, t);
+        for (unsigned w = 0; w < kWaves; ++w) t += wsum[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// *out = sum of part[0 .. n-1] (one workgroup of kSumThreads threads, each adding
+// independent loads).
+constexpr unsigned kSumThreads = 1024;
+__global__ __launch_bounds__(kSumThreads) void k_sum_parts(const unsigned long long* __restrict__ part, uint32_t n,
+                                                           unsigned long long* __restrict__ out) {
+    constexpr unsigned kWaves = kSumThreads / kWave;
+    __shared__ unsigned long long wsum[kWaves];
+    unsigned long long v[4] = {0, 0, 0, 0};
+    uint32_t i = threadIdx.x;
+    for (; i + 3 * kSumThreads < n; i += 4 * kSumThreads)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] += part[i + u * kSumThreads];
+    for (; i < n; i += kSumThreads) v[0] += part[i];
+    unsigned long long t = (v[0] + v[1]) + (v[2] + v[3]);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) t += __shfl_xor(t, d, kWave);
+    if (lane_id() == 0) wsum[threadIdx.x / kWave] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (unsigned w = 0; w < kWaves; ++w) s += wsum[w];
+        *out = s;
     }
 }
 
@@ -337,9 +381,13 @@ int extract_perm(lj_context* ctx, int64_t n, uint32_t* perm, hipStream_t s) {
 
 int count_candidates_from(lj_context* ctx, const GridGeom& g, const uint32_t* offsets, unsigned long long* out,
                           hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, out, 1u, 0ull);
-    const unsigned nb = div_up(g.ncells > 0 ? g.ncells : 1, 256);
-    hipLaunchKernelGGL(k_candidates, dim3(nb < kCandBlocks ? nb : kCandBlocks), dim3(256), 0, s, offsets, g, out);
+    const uint64_t nrows = (uint64_t)g.g[1] * (uint64_t)g.g[2];
+    const uint64_t rb = (nrows + 3) / 4;  // four waves (rows) per workgroup
+    const unsigned nb = (unsigned)(rb < kCandBlocks ? (rb > 0 ? rb : 1) : kCandBlocks);
+    LJ_HIP(ctx, ctx->bin.cand.grow((size_t)nb * sizeof(unsigned long long)));
+    unsigned long long* part = ctx->bin.cand.as<unsigned long long>();
+    hipLaunchKernelGGL(k_candidates, dim3(nb), dim3(256), 0, s, offsets, g, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kSumThreads), 0, s, (const unsigned long long*)part, nb, out);
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }

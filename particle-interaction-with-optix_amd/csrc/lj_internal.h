@@ -65,6 +65,7 @@ struct BinWork {
     DevBuf rows;         // u64 pair-list row offsets
     DevBuf ccounts;      // u32 counts of the cell_edge grid when the force grid is refined
     DevBuf coffsets;     // u32 offsets of the cell_edge grid (ncells+1)
+    DevBuf cand;         // u64 per-workgroup partial candidate counts
     int cur = 0;         // which keys buffer holds the sorted keys
     const uint32_t* sorted_keys() const { return keys[cur].as<uint32_t>(); }
 };
