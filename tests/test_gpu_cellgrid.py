@@ -45,6 +45,22 @@ def _case(name):
     if name == "one_cell_crowd":
         x, y, z = ljamd.generate("uniform", 11, 3000, [0.30, 0.31])
         return (x, y, z), ljamd.make_params(0.05)
+    if name == "gap_classes":
+        # one row of 131,072 cells of edge 2^-17 (exact keys): runs of empty cells of
+        # every length class of the offsets kernel (up to 8 cells: the lane; 9-256: the
+        # wave; longer: 4096-cell pieces), each class boundary on both sides, repeated
+        # cells, and a long empty stretch before the first and after the last key
+        rc = 2.0**-17
+        rng = np.random.default_rng(21)
+        cls = [1, 2, 8, 9, 10, 64, 255, 256, 257, 4095, 4096, 4097, 9000]
+        wts = np.array([8, 8, 8, 8, 8, 8, 8, 8, 8, 1, 1, 1, 1], float)
+        gaps = rng.choice(cls, size=1000, p=wts / wts.sum())
+        gaps[rng.random(gaps.size) < 0.2] = 0  # a second particle in the same cell
+        cells = 5000 + np.cumsum(gaps)
+        cells = cells[cells < (1 << 17) - 6000]
+        x = ((cells + 0.5) * rc).astype(np.float32)
+        y = np.full(x.size, 0.5 * rc, np.float32)
+        return (x, y, y.copy()), ljamd.make_params(rc, box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, rc, rc))
     if name == "sparse_26bit_keys":
         # 333^3 = 36,926,037 cells: a 26-bit key, sorted in three 9-bit passes
         x, y, z = ljamd.generate("uniform", 12, 200_000)
@@ -53,7 +69,7 @@ def _case(name):
 
 
 BIN_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7", "single", "empty", "edges", "one_cell_crowd",
-             "sparse_26bit_keys"]
+             "sparse_26bit_keys", "gap_classes"]
 
 
 @pytest.mark.parametrize("name", BIN_CASES)
@@ -83,7 +99,8 @@ def _forces(engine, x, y, z, p, energies=True):
     return f, e, res
 
 
-FORCE_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7", "single", "empty", "edges", "one_cell_crowd"]
+FORCE_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7", "single", "empty", "edges", "one_cell_crowd",
+               "gap_classes"]
 
 
 @pytest.mark.parametrize("name", FORCE_CASES)
@@ -109,7 +126,8 @@ def test_forces_mixed_vs_fp64(engine, name):
 
 
 @pytest.mark.parametrize("refine,refine_x", [(1, 1), (2, 2), (4, 4), (1, 4), (1, 8), (1, 16), (2, 8), (4, 16)])
-@pytest.mark.parametrize("name", ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "edges", "one_cell_crowd", "tiny_7"])
+@pytest.mark.parametrize("name", ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "edges", "one_cell_crowd", "tiny_7",
+                                  "gap_classes"])
 def test_refined_grids_same_pairs_and_forces(engine, name, refine, refine_x):
     """The 27-cell grid and the half/quarter-edge grids with stencil trimming visit
     every accepted pair: counts bit-exact, forces within the bar."""
