@@ -157,27 +157,48 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
     // a row are those of qx -+ w inv_cw, truncated (a negative lower bound truncates to
     // 0, like the clamp of the cell rule) and clipped to [x0, x1]
     const float qx = (pi.x - g.lo[0]) * tr.inv_cw;
-    for (int32_t zz = z0; zz <= z1; ++zz) {
-        const float gz = trim ? axis_gap(pi.z, zz, cz, g.lo[2], g.cw[2], tr.margin) : 0.f;
-        for (int32_t yy = y0; yy <= y1; ++yy) {
-            int32_t xa = x0, xb = x1;
-            if (trim) {
-                const float gy = axis_gap(pi.y, yy, cy, g.lo[1], g.cw[1], tr.margin);
-                const float w2 = tr.r2 - gz * gz - gy * gy;
-                if (w2 < 0.f) continue;
-                // hardware square root (about 1 ulp, no correctly rounded fix-up):
-                // its error is far inside the margin, which exceeds rc 2^-14
-                const float w = __builtin_amdgcn_sqrtf(w2) + tr.margin;
-                xa = min(max(x0, (int32_t)__builtin_fmaf(-w, tr.inv_cw, qx)), x1);
-                xb = min(x1, (int32_t)__builtin_fmaf(w, tr.inv_cw, qx));
+    // Slot range [b, en) of the trimmed x-cells of row (zz, yy); empty outside the
+    // clipped stencil or beyond the cutoff.
+    auto row_range = [&](int32_t zz, int32_t yy, uint32_t& b, uint32_t& en) {
+        b = en = 0u;
+        if (zz < z0 || zz > z1 || yy < y0 || yy > y1) return;
+        int32_t xa = x0, xb = x1;
+        if (trim) {
+            const float gz = axis_gap(pi.z, zz, cz, g.lo[2], g.cw[2], tr.margin);
+            const float gy = axis_gap(pi.y, yy, cy, g.lo[1], g.cw[1], tr.margin);
+            const float w2 = tr.r2 - gz * gz - gy * gy;
+            if (w2 < 0.f) return;
+            // hardware square root (about 1 ulp, no correctly rounded fix-up): its
+            // error is far inside the margin, which exceeds rc 2^-14
+            const float w = __builtin_amdgcn_sqrtf(w2) + tr.margin;
+            xa = min(max(x0, (int32_t)__builtin_fmaf(-w, tr.inv_cw, qx)), x1);
+            xb = min(x1, (int32_t)__builtin_fmaf(w, tr.inv_cw, qx));
+        }
+        const uint32_t row = cbase + (uint32_t)((zz * g.g[1] + yy) * g.g[0]);
+        b = off[row + xa];
+        en = off[row + xb + 1];
+    };
+    // Rows go in mirror pairs, (cz + dz, cy + dy) and then (cz - dz, cy - dy) for
+    // (dz, dy) in one half of the stencil, each pair walked as one stream of
+    // candidates.  A row and its mirror hold about as many candidates together wherever
+    // the particle sits in its cell, so the lanes of a wave finish a pair at closer
+    // times than a single row, and a wave runs each step for its longest lane.
+    for (int32_t dz = 0; dz <= R; ++dz)
+        for (int32_t dy = dz == 0 ? 0 : -R; dy <= R; ++dy) {
+            uint32_t t, en, b2, e2;
+            row_range(cz + dz, cy + dy, t, en);
+            if (dz == 0 && dy == 0) b2 = e2 = 0u;
+            else row_range(cz - dz, cy - dy, b2, e2);
+            if (t >= en) {
+                t = b2;
+                en = e2;
+                b2 = e2;
             }
-            const uint32_t row = cbase + (uint32_t)((zz * g.g[1] + yy) * g.g[0]);
-            const uint32_t b = off[row + xa], en = off[row + xb + 1];
             // candidate t + u exists iff t + u < en, i.e. t < enu[u] = max(en - u, 0)
             uint32_t enu[kCandidates];
 #pragma unroll
             for (int u = 1; u < kCandidates; ++u) enu[u] = en > (uint32_t)u ? en - (uint32_t)u : 0u;
-            for (uint32_t t = b; t < en; t += kCandidates) {
+            while (t < en) {
                 if constexpr (QUEUE)
                     if (__any(pc + kCandidates > depth)) flush();
                 // slots up to t + kCandidates - 1 <= n + kCandidates - 2 exist (pos has
@@ -191,9 +212,16 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
 #pragma unroll
                 for (int u = 1; u < kCandidates; ++u)
                     if (t < enu[u]) visit(t + u, pj[u]);
+                t += kCandidates;
+                if (t >= en) {  // on to the mirror row (then t = en = e2 ends the pair)
+                    t = b2;
+                    en = e2;
+                    b2 = e2;
+#pragma unroll
+                    for (int u = 1; u < kCandidates; ++u) enu[u] = en > (uint32_t)u ? en - (uint32_t)u : 0u;
+                }
             }
         }
-    }
     // rows end at different times in different lanes: flush while any queue holds pairs
     if constexpr (QUEUE)
         while (__any(pc > 0)) flush();
