@@ -4,7 +4,9 @@
 // Reference: ComputeParticleInterationsParPartNoLoop (Cuda-version/cudaLJ.cu:299-341)
 // visits, for one particle, the particles of the clipped 27 neighbour cells in z -> y
 // -> x order and skips itself.  Cell order is z-major (cudaLJ.cu:127-134), so the
-// x-cells of one (y, z) row of the stencil are one contiguous slot range.
+// x-cells of one (y, z) row of the stencil are one contiguous slot range.  The rows are
+// visited here in another order (mirror pairs, see stencil_pairs): the pair set is the
+// same, and the FP64 sums differ only in rounding.
 #pragma once
 
 #include "lj_internal.h"
@@ -84,8 +86,8 @@ __device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const
 
 // Accepted partners of the particle in sorted slot k (position pi, cell `key` of grid
 // g, whose cell 0 is global cell `cbase`), summed into acc; returns their number.
-// The stencil is (2Rx+1) x (2R+1)^2 cells of g, visited as (2R+1)^2 x-rows in z -> y
-// order, rows and x-cells beyond the cutoff trimmed.
+// The stencil is (2Rx+1) x (2R+1)^2 cells of g, visited as (2R+1)^2 x-rows in mirror
+// pairs (below), rows and x-cells beyond the cutoff trimmed.
 //
 // Pairs.  With QUEUE = false (the default shape) the FP64 pair arithmetic of an
 // accepted candidate runs inside the candidate loop; it then runs for the whole wave
