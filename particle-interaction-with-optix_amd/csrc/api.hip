@@ -189,7 +189,7 @@ static void release_all(lj_context* ctx) {
     for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags})
         d->release();
     BatchWork& t = ctx->batch;
-    for (DevBuf* d : {&t.sys, &t.hints, &t.npair, &t.ev}) d->release();
+    for (DevBuf* d : {&t.sys, &t.hints, &t.npair, &t.ev, &t.wrec}) d->release();
     ctx->tune_out.release();
 }
 

@@ -127,27 +127,26 @@ __global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __rest
     }
 }
 
+// Energy, virial and accepted pairs of the 64 sorted slots of one wave, when they all
+// belong to system `sys`; sys = kSharedWave marks a wave whose slots belong to several
+// systems (their values are then kept per slot).
+struct WaveTotals {
+    double e, v;
+    unsigned long long np;
+    uint32_t sys, pad;
+};
+constexpr uint32_t kSharedWave = 0xffffffffu;
+
 // One thread per sorted slot, as the single-system kernel (the same stencil loop and
-// LDS queues), with the grid, trimming and pair constants of the slot's system.  A
-// wave may span several small systems: each lane reads its own system's constants.
-// Per-particle energy, virial and pair count go to arrays in the caller's order for
-// the per-system reduction.
+// LDS queues), with the grid, trimming and pair constants of the slot's system d:
+// forces and energies to the caller's order, the slot's energy, virial and accepted
+// pairs to ei, vi, np (zero for an idle lane).  Every lane of the wave calls it.
 template <int PRED, int PREC, bool QUEUE>
-__global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restrict__ pos,
-                                                           const uint32_t* __restrict__ cell_of,
-                                                           const uint32_t* __restrict__ off, uint32_t n,
-                                                           const SysDesc* __restrict__ sys,
-                                                           const uint32_t* __restrict__ cbase,
-                                                           const uint32_t* __restrict__ chint, uint32_t nsys, int Rx,
-                                                           int R,
-                                                           uint32_t depth, float* __restrict__ fx, float* __restrict__ fy,
-                                                           float* __restrict__ fz, float* __restrict__ e,
-                                                           double2* __restrict__ ev, uint32_t* __restrict__ npair) {
-    extern __shared__ uint32_t queue[];
-    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const bool active = k < n;
-    const uint32_t key = active ? cell_of[k] : 0u;
-    const SysDesc& d = sys[find_sys(cbase, chint, nsys, key)];
+__device__ inline void batch_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
+                                   bool active, uint32_t key, const SysDesc& d, int Rx, int R, uint32_t depth,
+                                   uint32_t* __restrict__ queue, uint32_t stride, float* __restrict__ fx,
+                                   float* __restrict__ fy, float* __restrict__ fz, float* __restrict__ e, double& ei,
+                                   double& vi, uint32_t& np) {
     const GridGeom g = d.g;
     const PairConst c = d.c;
     const Trim tr = d.tr;
@@ -156,33 +155,109 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
     acc.allow_exact_diff(c, pi, active);
-    const uint32_t np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, Rx, R, tr, c,
-                                                        queue + threadIdx.x, blockDim.x, depth, acc);
-    if (!active) return;
-    const uint32_t i = __float_as_uint(pi.w);
-    fx[i] = (float)(-c.e24 * (double)acc.ax.value());
-    fy[i] = (float)(-c.e24 * (double)acc.ay.value());
-    fz[i] = (float)(-c.e24 * (double)acc.az.value());
-    const double ei = c.e4 * (double)acc.ae.value();
-    if (e) e[i] = (float)ei;
-    ev[i] = make_double2(ei, c.e24 * (double)acc.av.value());
-    npair[i] = np;
+    np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, Rx, R, tr, c, queue, stride,
+                                                depth, acc);
+    ei = vi = 0.0;
+    if (active) {
+        const uint32_t i = __float_as_uint(pi.w);
+        fx[i] = (float)(-c.e24 * (double)acc.ax.value());
+        fy[i] = (float)(-c.e24 * (double)acc.ay.value());
+        fz[i] = (float)(-c.e24 * (double)acc.az.value());
+        ei = c.e4 * (double)acc.ae.value();
+        if (e) e[i] = (float)ei;
+        vi = c.e24 * (double)acc.av.value();
+    }
 }
 
-// One block per system: FP64 totals over its particles in index order.
-__global__ __launch_bounds__(256) void k_batch_reduce(const SysDesc* __restrict__ sys, const double2* __restrict__ ev,
+// The waves whose 64 sorted slots belong to one system (nearly all of them): that
+// system's constants are read once per wave (wave-uniform, so they sit in scalar
+// registers, not in every lane), and the wave adds its energies, virials and pair
+// counts (fixed butterfly order) into one WaveTotals.  A wave shared by several
+// systems returns at once; k_batch_force_shared serves it.  A system's sorted slots are
+// the caller's index range [first, first + n) of that system (the key space orders
+// the systems).
+template <int PRED, int PREC, bool QUEUE>
+__global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restrict__ pos,
+                                                           const uint32_t* __restrict__ cell_of,
+                                                           const uint32_t* __restrict__ off, uint32_t n,
+                                                           const SysDesc* __restrict__ sys,
+                                                           const uint32_t* __restrict__ cbase,
+                                                           const uint32_t* __restrict__ chint, uint32_t nsys, int Rx,
+                                                           int R, uint32_t depth, float* __restrict__ fx,
+                                                           float* __restrict__ fy, float* __restrict__ fz,
+                                                           float* __restrict__ e, WaveTotals* __restrict__ wrec) {
+    extern __shared__ uint32_t queue[];
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const bool active = k < n;
+    const uint32_t key = active ? cell_of[k] : 0u;
+    const uint32_t si = find_sys(cbase, chint, nsys, key);
+    // lane 0 holds the wave's lowest slot: idle only when the whole wave is
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(active ? si : kSharedWave);
+    if (s0 == kSharedWave || !__all(!active || si == s0)) return;
+    double ei, vi;
+    uint32_t np;
+    batch_pairs<PRED, PREC, QUEUE>(pos, off, k, active, key, sys[s0], Rx, R, depth, queue + threadIdx.x, blockDim.x, fx,
+                                   fy, fz, e, ei, vi, np);
+    const double es = wave_sum(ei), vs = wave_sum(vi);
+    const unsigned long long ns = wave_sum((unsigned long long)np);
+    if (__lane_id() == 0) wrec[k / kWaveSize] = WaveTotals{es, vs, ns, s0, 0u};
+}
+
+// The waves shared by several systems (listed in the plan; at most one per system
+// boundary), one per workgroup: each lane reads its own system's constants, and the
+// slots' energies, virials and pair counts are kept per slot.
+template <int PRED, int PREC, bool QUEUE>
+__global__ __launch_bounds__(kWaveSize) void k_batch_force_shared(
+    const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of, const uint32_t* __restrict__ off, uint32_t n,
+    const SysDesc* __restrict__ sys, const uint32_t* __restrict__ cbase, const uint32_t* __restrict__ chint,
+    uint32_t nsys, const uint32_t* __restrict__ shared_waves, int Rx, int R, uint32_t depth, float* __restrict__ fx,
+    float* __restrict__ fy, float* __restrict__ fz, float* __restrict__ e, double2* __restrict__ ev,
+    uint32_t* __restrict__ npair, WaveTotals* __restrict__ wrec) {
+    extern __shared__ uint32_t queue[];
+    const uint32_t w = shared_waves[blockIdx.x];
+    const uint32_t k = w * kWaveSize + threadIdx.x;
+    const bool active = k < n;
+    const uint32_t key = active ? cell_of[k] : 0u;
+    double ei, vi;
+    uint32_t np;
+    batch_pairs<PRED, PREC, QUEUE>(pos, off, k, active, key, sys[find_sys(cbase, chint, nsys, key)], Rx, R, depth,
+                                   queue + threadIdx.x, blockDim.x, fx, fy, fz, e, ei, vi, np);
+    if (active) {
+        ev[k] = make_double2(ei, vi);
+        npair[k] = np;
+    }
+    if (threadIdx.x == 0) wrec[w] = WaveTotals{0.0, 0.0, 0ull, kSharedWave, 0u};
+}
+
+// One block per system: FP64 totals over its waves in slot order -- the WaveTotals of
+// the waves it fills alone, the per-slot values of its slots in the (at most two)
+// waves it shares with other systems.
+__global__ __launch_bounds__(256) void k_batch_reduce(const SysDesc* __restrict__ sys,
+                                                      const WaveTotals* __restrict__ wrec,
+                                                      const double2* __restrict__ ev,
                                                       const uint32_t* __restrict__ npair, lj_result* __restrict__ res) {
     const SysDesc& d = sys[blockIdx.x];
     double a = 0.0, b = 0.0;
     unsigned long long np = 0;
-    // unrolled so that the loads of several iterations are in flight; each thread still
-    // adds its values in index order
-#pragma unroll 8
-    for (uint32_t q = threadIdx.x; q < d.n; q += 256) {
-        const double2 v = ev[d.first + q];
-        a += v.x;
-        b += v.y;
-        np += npair[d.first + q];
+    if (d.n) {
+        const uint32_t w0 = d.first / kWaveSize, w1 = (d.first + d.n - 1) / kWaveSize;
+        for (uint32_t w = w0 + threadIdx.x; w <= w1; w += 256) {
+            const WaveTotals r = wrec[w];
+            if (r.sys != kSharedWave) {
+                a += r.e;
+                b += r.v;
+                np += r.np;
+            } else {
+                const uint32_t lo = d.first > w * kWaveSize ? d.first : w * kWaveSize;
+                const uint32_t hi = d.first + d.n < (w + 1) * kWaveSize ? d.first + d.n : (w + 1) * kWaveSize;
+                for (uint32_t q = lo; q < hi; ++q) {
+                    const double2 t = ev[q];
+                    a += t.x;
+                    b += t.y;
+                    np += npair[q];
+                }
+            }
+        }
     }
     block_sum2(a, b);
     const unsigned long long t = block_sum_u64(np);
@@ -251,6 +326,19 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     }
     if (ntot > 0 && (!x || !y || !z || !fx || !fy || !fz))
         return fail(ctx, LJ_ERR_INVALID_ARG, "positions and forces must not be NULL");
+    // waves of 64 sorted slots holding slots of more than one system: the wave of each
+    // boundary between non-empty systems that does not fall on a wave boundary
+    std::vector<uint32_t> shared;
+    {
+        bool seen = false;
+        for (int32_t s = 0; s < nsys; ++s) {
+            if (!desc[(size_t)s].n) continue;
+            const uint32_t f = desc[(size_t)s].first;
+            if (seen && f % (uint32_t)kWaveSize && (shared.empty() || shared.back() != f / (uint32_t)kWaveSize))
+                shared.push_back(f / (uint32_t)kWaveSize);
+            seen = true;
+        }
+    }
     LJ_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)stream;
     BatchWork& W = ctx->batch;
@@ -259,13 +347,15 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     // Upload the plan only when it changed (a cached plan keeps replays free of
     // host-to-device copies).
     const size_t dbytes = desc.size() * sizeof(SysDesc), ubytes = (size_t)nsys * 4;
-    std::vector<uint8_t> blob(dbytes + 3 * ubytes);
+    const size_t sbytes = shared.size() * 4;
+    std::vector<uint8_t> blob(dbytes + 3 * ubytes + sbytes);
     std::memcpy(blob.data(), desc.data(), dbytes);
     std::memcpy(blob.data() + dbytes, first.data(), ubytes);
     std::memcpy(blob.data() + dbytes + ubytes, cbase.data(), ubytes);
     std::memcpy(blob.data() + dbytes + 2 * ubytes, ccbase.data(), ubytes);
+    if (sbytes) std::memcpy(blob.data() + dbytes + 3 * ubytes, shared.data(), sbytes);
     const SysDesc* dsys = nullptr;
-    const uint32_t *dfirst = nullptr, *dcbase = nullptr, *dccbase = nullptr;
+    const uint32_t *dfirst = nullptr, *dcbase = nullptr, *dccbase = nullptr, *dshared = nullptr;
     // search hints of the three base arrays (particles, force-grid cells, cell_edge cells)
     const size_t h1 = hint_entries(ntot), h2 = hint_entries(ctot), h3 = hint_entries(cctot);
     auto bind = [&]() {
@@ -274,6 +364,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         dfirst = reinterpret_cast<const uint32_t*>(dplan + dbytes);
         dcbase = reinterpret_cast<const uint32_t*>(dplan + dbytes + ubytes);
         dccbase = reinterpret_cast<const uint32_t*>(dplan + dbytes + 2 * ubytes);
+        dshared = reinterpret_cast<const uint32_t*>(dplan + dbytes + 3 * ubytes);
     };
     if (blob != W.plan) {
         LJ_HIP(ctx, W.sys.grow(blob.size()));
@@ -302,6 +393,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     const size_t nn = (size_t)(n > 0 ? n : 1);
     LJ_HIP(ctx, W.ev.grow(nn * sizeof(double2)));
     LJ_HIP(ctx, W.npair.grow(nn * sizeof(uint32_t)));
+    LJ_HIP(ctx, W.wrec.grow(div_up(nn, (size_t)kWaveSize) * sizeof(WaveTotals)));
     uint32_t* off;
     {
         PhaseTimer tb(ctx, LJ_PHASE_BIN, s);
@@ -330,18 +422,26 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     }
     double2* ev = W.ev.as<double2>();
     uint32_t* npair = W.npair.as<uint32_t>();
+    WaveTotals* wrec = W.wrec.as<WaveTotals>();
     const float4* pos = B.pos.as<float4>();
     const uint32_t* cell_of = B.sorted_keys();
     {
         PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
         const unsigned nb = div_up(n, sh.block);
         const size_t lds = sh.lds_bytes();
+        const size_t lds_shared = sh.queued() ? (size_t)sh.depth * kWaveSize * sizeof(uint32_t) : 0;
         if (n > 0) {
 #define LJ_BFQ(PRED, PREC, Q)                                                                                     \
-    hipLaunchKernelGGL((k_batch_force<PRED, PREC, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of,            \
-                       (const uint32_t*)off, (uint32_t)n, dsys, dcbase, hcbase, (uint32_t)nsys, sh.refine_x, sh.refine, \
-                       (uint32_t)sh.depth, \
-                       fx, fy, fz, e, ev, npair)
+    do {                                                                                                          \
+        hipLaunchKernelGGL((k_batch_force<PRED, PREC, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of,        \
+                           (const uint32_t*)off, (uint32_t)n, dsys, dcbase, hcbase, (uint32_t)nsys, sh.refine_x,    \
+                           sh.refine, (uint32_t)sh.depth, fx, fy, fz, e, wrec);                                     \
+        if (!shared.empty())                                                                                      \
+            hipLaunchKernelGGL((k_batch_force_shared<PRED, PREC, Q>), dim3((unsigned)shared.size()),              \
+                               dim3(kWaveSize), lds_shared, s, pos, cell_of, (const uint32_t*)off, (uint32_t)n,    \
+                               dsys, dcbase, hcbase, (uint32_t)nsys, dshared, sh.refine_x, sh.refine,              \
+                               (uint32_t)sh.depth, fx, fy, fz, e, ev, npair, wrec);                                 \
+    } while (0)
 #define LJ_BF(PRED, PREC)                          \
     do {                                           \
         if (sh.queued()) LJ_BFQ(PRED, PREC, true); \
@@ -367,8 +467,8 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     }
     {
         PhaseTimer to(ctx, LJ_PHASE_OTHER, s);
-        hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)nsys), dim3(256), 0, s, dsys, (const double2*)ev,
-                           (const uint32_t*)npair, results);
+        hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)nsys), dim3(256), 0, s, dsys, (const WaveTotals*)wrec,
+                           (const double2*)ev, (const uint32_t*)npair, results);
     }
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;

@@ -84,8 +84,9 @@ struct LbvhWork {
 struct BatchWork {
     DevBuf sys;           // device copy of the plan: per-system descriptors and cell bases
     DevBuf hints;         // search hints of the plan's three base arrays (rebuilt with the plan)
-    DevBuf npair;         // u32 accepted-pair count of each particle (caller order)
-    DevBuf ev;            // double2 energy and virial of each particle (caller order)
+    DevBuf npair;         // u32 accepted-pair count of each sorted slot in a wave shared by systems
+    DevBuf ev;            // double2 energy and virial of each sorted slot in a wave shared by systems
+    DevBuf wrec;          // one WaveTotals per wave of 64 sorted slots
     std::vector<uint8_t> plan;  // host copy of the plan last uploaded to sys
 };
 

@@ -120,6 +120,47 @@ def test_batched_equals_single_calls_and_oracle(engine, refine, refine_x):
         assert err.max() <= 1e-5, s
 
 
+def test_batched_small_and_empty_systems(engine):
+    """Systems smaller than a wave of 64 sorted slots, several in one wave, empty ones,
+    and boundaries on and off the wave boundaries: each system gets the bits and
+    counts of its own call and the oracle's counts and totals."""
+    sizes = [0, 1, 5, 63, 64, 65, 0, 2, 130, 7, 64, 1, 0, 200, 3, 61, 64, 128, 1, 1, 0]
+    rng = np.random.default_rng(19)
+    systems, parts, first = [], [], 0
+    for s, n in enumerate(sizes):
+        if n:
+            x, y, z = ljamd.generate("uniform", 500 + s, n)
+        else:
+            x = y = z = np.zeros(0, np.float32)
+        parts.append((x, y, z))
+        # about 4-20 neighbours per particle
+        rc = float(0.35 + 0.25 * rng.random()) if n < 20 else float((3.0 * 12 / (4 * np.pi * n)) ** (1 / 3))
+        systems.append(ljamd.LJSystem(first, n, ljamd.make_params(rc)))
+        first += n
+    X, Y, Z = (np.concatenate([q[i] for q in parts]).astype(np.float32) for i in range(3))
+    print(f"[batched small] systems={len(sizes)} n={X.size}", flush=True)
+    dx, dy, dz = engine.to_device(X, Y, Z)
+    out = engine.compute_batched(systems, dx, dy, dz)
+    engine.torch.cuda.synchronize()
+    res = ljamd.read_results(out["result"], len(systems))
+    fb = [out[k][: X.size].cpu().numpy() for k in ("fx", "fy", "fz", "e")]
+    for s, sy in enumerate(systems):
+        a, b = sy.first, sy.first + sy.n
+        ref = po.forces_f64(po.from_lj(sy.params), X[a:b], Y[a:b], Z[a:b])
+        assert int(res[s]["n_pairs"]) == ref["n_pairs"], s
+        assert int(res[s]["n_candidates"]) == ref["n_candidates"], s
+        assert res[s]["e_total"] == pytest.approx(ref["e_total"], rel=1e-9, abs=1e-12), s
+        assert res[s]["virial"] == pytest.approx(ref["virial"], rel=1e-9, abs=1e-12), s
+        if sy.n == 0:
+            continue
+        single = engine.compute(dx[a:b], dy[a:b], dz[a:b], sy.params)
+        engine.torch.cuda.synchronize()
+        r1 = ljamd.read_results(single["result"])[0]
+        assert int(r1["n_pairs"]) == int(res[s]["n_pairs"]), s
+        for k, arr in zip(("fx", "fy", "fz", "e"), fb):
+            assert np.array_equal(single[k][: sy.n].cpu().numpy().view(np.uint32), arr[a:b].view(np.uint32)), (s, k)
+
+
 def test_batched_autotune(engine):
     """lj_autotune_batched writes one valid shape into every system; the tuned batch
     finds the same pairs as the default one."""
