@@ -8,8 +8,12 @@ compute phases, Cuda-version/cudaLJ.cu:613-653, plus forces).  value = accepted
 ordered pairs / step time.  The JSON line also carries:
   roofline      the force kernel's FP32-vector roofline fraction: 25 flops per
                 accepted ordered pair over its hipEvent-timed duration
-  cpu_baseline  the oracle's FP32 cell-grid restatement on the host cores, on a
-                same-density sample (bounded to a few seconds of CPU work)
+  accuracy      max / RMS relative and conditioned force error, energy error and
+                non-finite counts of the last timed step against the FP64 oracle on a
+                65,536-particle subsample (the oracle as the checker)
+  cpu_baseline  the oracle's FP32 cell-grid restatement of the reference path on the
+                whole workload with the host threads this job may use (plus a
+                one-thread figure on a sample), with the CPU model
   batched       the 4,096-system batch: one call over all systems (eager and
                 replayed from a captured graph) against one call per system
 Before the timed steps the kernel shape (force-grid refinement, particles per
@@ -51,48 +55,120 @@ def time_steps(torch, fn, steps):
     return start.elapsed_time(stop) / steps
 
 
-def traffic_of(kernel_prefix: str, workload: str):
-    """HBM bytes per dispatch of the force kernel from the committed PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+def traffic_of(kernel_prefix: str, workload: str, p):
+    """HBM bytes per dispatch of the force kernel in the shape being timed, from the
+    committed PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when that shape was not profiled."""
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
     try:
         d = json.loads(f.read_text())
-        return d.get(workload, {}).get(kernel_prefix)
+        shape = f"{p.refine},{p.refine_x},{p.block},{p.depth}"
+        v = d.get(workload, {}).get(kernel_prefix)
+        return v.get(shape) if isinstance(v, dict) else None
     except Exception:
         return None
 
 
-def cpu_baseline(ljamd, workload: str):
+def _cpu_info():
+    """Host CPU description: logical CPUs the machine shows, the CPUs this process may
+    run on, and the model name."""
+    model = None
+    try:
+        for ln in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity": affinity, "model": model}
+
+
+def oracle_leg(ljamd, workload: str, w, out, max_threads: int):
+    """The CPU side of the benchmark, where the oracle is the checker and the timed
+    baseline:
+      accuracy      the GPU forces of the last timed step against the FP64 oracle on a
+                    fixed subsample (every 256th particle of C3: 65,536 rows)
+      cpu_baseline  the oracle's FP32 cell-grid restatement of the reference path, timed
+                    on the full workload with every host thread this job may use, plus a
+                    one-thread run on a bounded sample."""
+    import numpy as np
+
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(16, cores))
-    if workload == "c3":
-        # 1/8 of the unit box at C3's density: 2,097,152 particles in [0, 0.5)^3
-        n, hi = 2_097_152, 0.5
-        x, y, z = ljamd.generate("uniform", 1236, n, [0.0, hi])
-        p = po.params(0.01, box_hi=(hi, hi, hi))
-        sample = f"N={n} uniform in [0,0.5)^3 (C3 density), rc=0.01, full bin+scan+sort+27-cell force"
-    else:
-        from ljamd import workloads
-
-        w = workloads.single(workload)
-        x, y, z = w.x, w.y, w.z
-        p = po.from_lj(w.params)
-        sample = w.desc
-    po.forces_f32(p, x[:1000], y[:1000], z[:1000], nthreads=threads)  # warm the pool
+    info = _cpu_info()
+    threads = max(1, min(max_threads, info["affinity"] or 1))
+    q = po.from_lj(w.params)
+    n = w.x.size
+    # ---- accuracy of the GPU result
+    t0 = time.perf_counter()
+    step = max(1, n // 65536)
+    sub = np.arange(0, n, step, dtype=np.int64)
+    ref = po.forces_f64(q, w.x, w.y, w.z, subset=sub, nthreads=threads)
+    f = [out[k][:n].cpu().numpy()[sub].astype(np.float64) for k in ("fx", "fy", "fz")]
+    e = out["e"][:n].cpu().numpy()[sub].astype(np.float64)
+    g = [ref["fx"], ref["fy"], ref["fz"]]
+    flt_max = float(np.finfo(np.float32).max)
+    fin64 = (np.abs(g[0]) <= flt_max) & (np.abs(g[1]) <= flt_max) & (np.abs(g[2]) <= flt_max)
+    fin32 = np.isfinite(f[0]) & np.isfinite(f[1]) & np.isfinite(f[2])
+    ok = fin64 & fin32
+    diff = np.sqrt(sum((a[ok] - b[ok]) ** 2 for a, b in zip(f, g)))
+    mag = np.sqrt(sum(b[ok] ** 2 for b in g))
+    rel = np.where(mag > 0, diff / np.where(mag > 0, mag, 1.0), diff)
+    cond = np.where(ref["cond"][ok] > 0, diff / np.where(ref["cond"][ok] > 0, ref["cond"][ok], 1.0), diff)
+    efin = np.isfinite(e) & (np.abs(ref["e"]) <= flt_max)
+    erel = np.abs(e[efin] - ref["e"][efin]) / np.maximum(np.abs(ref["e"][efin]), 1e-300)
+    accuracy = {
+        "reference": "FP64 oracle (orc_forces_f64) on the same FP32 positions",
+        "rows": int(sub.size),
+        "sample": f"every {step}th particle",
+        "max_rel_force_err": float(rel.max()) if rel.size else 0.0,
+        "rms_rel_force_err": float(np.sqrt(np.mean(rel**2))) if rel.size else 0.0,
+        "max_cond_force_err": float(cond.max()) if cond.size else 0.0,
+        "rms_cond_force_err": float(np.sqrt(np.mean(cond**2))) if cond.size else 0.0,
+        "max_rel_energy_err": float(erel.max()) if erel.size else 0.0,
+        "nonfinite_fp64_forces": int((~fin64).sum()),
+        "gpu_nonfinite_where_fp64_finite": int((fin64 & ~fin32).sum()),
+        "within_1e-5": bool(rel.size == 0 or rel.max() <= 1e-5),
+        "seconds": round(time.perf_counter() - t0, 1),
+    }
+    log(f"accuracy {accuracy}")
+    # ---- CPU baseline: full workload on `threads` threads, then one thread on a sample
+    po.forces_f32(q, w.x[:1000], w.y[:1000], w.z[:1000], nthreads=threads)  # warm the pool
     reps, t_total, pairs = 0, 0.0, 0
-    while t_total < 8.0 and reps < 10:
+    while (t_total < 6.0 and reps < 3) or reps == 0:
         t0 = time.perf_counter()
-        r = po.forces_f32(p, x, y, z, nthreads=threads)
+        r = po.forces_f32(q, w.x, w.y, w.z, nthreads=threads)
         t_total += time.perf_counter() - t0
         pairs += r["n_pairs"]
         reps += 1
-    return {"value": pairs / t_total, "unit": "pair interactions/s", "cores": threads, "kind": "port",
-            "sample": sample + f", {reps} reps, {t_total:.1f} s"}
+    full = pairs / t_total
+    if workload == "c3":
+        # one thread: 1/8 of the unit box at C3's density (2,097,152 particles in [0, 0.5)^3)
+        ns, hi = 2_097_152, 0.5
+        x1, y1, z1 = ljamd.generate("uniform", 1236, ns, [0.0, hi])
+        q1 = po.params(0.01, box_hi=(hi, hi, hi))
+        s1 = f"N={ns} uniform in [0,0.5)^3 (C3 density)"
+    else:
+        ns = min(n, 262_144)
+        x1, y1, z1, q1 = w.x[:ns], w.y[:ns], w.z[:ns], q
+        s1 = f"first {ns} particles of the workload"
+    t0 = time.perf_counter()
+    r1 = po.forces_f32(q1, x1, y1, z1, nthreads=1)
+    one = r1["n_pairs"] / (time.perf_counter() - t0)
+    return accuracy, {
+        "value": full,
+        "unit": "pair interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{w.desc}: the whole workload, bin + scan + sort + 27-cell FP32 force, {reps} reps, "
+                  f"{t_total:.1f} s, OpenMP on {threads} threads",
+        "cpu": info,
+        "one_core": {"value": one, "sample": s1 + ", 1 thread"},
+    }
 
 
 def batched(torch, ljamd, steps, autotune=True):
@@ -177,7 +253,10 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp32_comp"])
     ap.add_argument("--lbvh", action="store_true", help="time the LBVH path instead of the cell grid")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU leg (the oracle's accuracy check and the timed CPU baseline)")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16),
+                    help="host threads for the CPU leg (default: OMP_NUM_THREADS, else 16)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="use the default kernel shape")
     ap.add_argument("--shape", default=None, metavar="REFINE,REFINE_X,BLOCK,DEPTH",
@@ -253,8 +332,10 @@ def main():
             "autotune": tuning,
         },
         "roofline": {
-            "bound": "mfma",
-            "note": "FP32 peak 157.3 TFLOP/s (the FP32 vector and matrix figure); the kernel runs on VALU; "
+            "bound": "valu",
+            "note": "vector-ALU bound, no matrix instructions: 25 flops per accepted ordered pair over the "
+                    "kernel's hipEvent time, against the 157.3 TFLOP/s FP32 vector peak (which assumes packed "
+                    "FP32; this build issues unpacked instructions only, so 50% of it is the ceiling). "
                     "frac_fp64_peak: the same flops against the 78.6 TFLOP/s FP64 vector peak, the precision "
                     "of the accepted-pair arithmetic in the mixed mode",
             "achieved": achieved,
@@ -262,10 +343,9 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "frac_fp64_peak": achieved / PEAK_FP64_TFLOPS,
-            "traffic": traffic_of(kernel, args.workload),
+            "traffic": traffic_of(kernel, args.workload, w.params),
         },
     }
-    eng.close()
     if args.calibration_copy:
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -277,8 +357,10 @@ def main():
         del src, dst
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        line["cpu_baseline"] = cpu_baseline(ljamd, args.workload)
+        line["accuracy"], line["cpu_baseline"] = oracle_leg(ljamd, args.workload, w, out, args.cpu_threads)
         log(f"cpu baseline {time.perf_counter() - t0:.1f} s")
+    del out
+    eng.close()
     if not args.no_batched:
         line["batched"] = batched(torch, ljamd, args.steps, autotune=not args.no_autotune)
     print(json.dumps(line), flush=True)

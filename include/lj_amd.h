@@ -45,8 +45,17 @@ extern "C" {
  *   CUSTOM   SDK/optixMultiCustom/optixMultiCustomAABB.cu:59   1e-16 <= d2 < rc2
  *   CUDALJ   Cuda-version/cudaLJ.cu:319-337  every 27-cell candidate j != i, no cutoff,
  *            r2 = d2 + softening2 (cudaLJ.cu:28-33)
- *   SPHERE   SDK/optixMultiSphere/optixLJ.cu:166      0 < d2 < rc2
- *   TRIANGLE SDK/optixMultiTriangle/optixLJ.cu:188    1e-4 < sqrt(d2) < rc */
+ *   SPHERE   ray-hit model of SDK/optixMultiSphere/optixLJ.cu:77-194 (optixSphere.cpp:156-157):
+ *            d2 < rc2 and particle i's axis ray closest to p_j (ties x, y, z), cast from
+ *            p_i - h e_a over [0, 2h], h = rc*0.81649658092f, reaches the sphere of radius
+ *            h + 1e-5f around p_j (near-coincident pairs are missed)
+ *   TRIANGLE ray-hit model of SDK/optixMultiTriangle/optixLJ.cu:82-217 (optixTriangle.cpp:
+ *            184-222): four +x rays of particle i and two panels of particle j; each
+ *            ray-panel hit with 1e-4 < |q - p_i| < rc (q the panel's rebuilt centre) that
+ *            passes the ray-index rule adds the pair, so a pair can count 2 or 4 times
+ * For SPHERE and TRIANGLE a pair's multiplicity (the reference's number of any-hit
+ * calls that add it) weights its force and energy terms and counts in n_pairs; the
+ * neighbour-list calls list each partner with a multiplicity above 0 once. */
 #define LJ_PRED_AABB 0
 #define LJ_PRED_CUSTOM 1
 #define LJ_PRED_CUDALJ 2
@@ -66,6 +75,10 @@ extern "C" {
 
 /* lj_params.flags */
 #define LJ_FLAG_NO_VIRIAL 1u /* skip the virial sum */
+/* Testing aid for the LBVH calls: caps every tree walk at one step, so that the walk's
+ * step guard trips and the calls take their error path (lj_lbvh_compute marks its
+ * results, lj_lbvh_pairs returns LJ_ERR_OVERFLOW).  No effect on the other calls. */
+#define LJ_FLAG_WALK_GUARD_TEST 2u
 
 
 
@@ -104,7 +117,7 @@ typedef struct lj_result {
     double e_total;                  /* 1/2 sum_i e_i, FP64 reduction */
     double virial;                   /* 1/2 sum_i sum_j w_ij r_ij^2, FP64 reduction */
     unsigned long long n_candidates; /* 27-cell candidates, ComputeNbInteractions semantics */
-    unsigned long long n_pairs;      /* accepted ordered pairs */
+    unsigned long long n_pairs;      /* accepted ordered pairs (SPHERE, TRIANGLE: summed multiplicities) */
 } lj_result;
 
 typedef struct lj_context lj_context;
@@ -121,6 +134,15 @@ int lj_context_destroy(lj_context* ctx);
 /* Message of the last failing call on ctx (NULL ctx: the last failing
  * lj_context_create in this thread). */
 const char* lj_last_error(const lj_context* ctx);
+
+/* Host: the context's workspace generation, which changes whenever a call grows
+ * (reallocates) any workspace buffer.  No reference counterpart (the reference
+ * allocates and frees its buffers on every run, cudaLJ.cu:554-580, 474-497): a
+ * captured graph holds workspace addresses, so a caller records the generation after
+ * capturing and replays only while it is unchanged.  Testing aid: the environment
+ * variable LJ_TEST_FAIL_ALLOC_AFTER=k, read by lj_context_create, lets k more device
+ * allocations succeed in the process and makes every later one fail with LJ_ERR_OOM. */
+int lj_workspace_generation(lj_context* ctx, uint64_t* generation);
 
 /* Host: grid dimensions and cell count for params (cudaLJ.cu:525-531). */
 int lj_grid_dims(const lj_params* p, int32_t dims[3], int64_t* n_cells);
@@ -172,7 +194,10 @@ int lj_pairs(lj_context* ctx, const float* x, const float* y, const float* z, in
  * the software counterpart of optixAccelBuild + optixLaunch with the
  * __intersection__sphere program (SDK/optixMultiSphereAABB/optixSphere.cpp:204-289,
  * 511-520; optixLJ.cu:49-88,134-176).  Same outputs as lj_compute except
- * n_candidates (leaf boxes tested). */
+ * n_candidates (leaf boxes tested).  Every walk is guarded by a step limit (2n steps,
+ * which a well-formed tree never needs); if one trips, the call still returns LJ_OK
+ * (it does not synchronise) but marks result: n_pairs = UINT64_MAX and NaN e_total
+ * and virial, so reading the result shows the failure. */
 int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
                     const lj_params* p, float* fx, float* fy, float* fz, float* e,
                     lj_result* result, void* stream);

@@ -200,6 +200,145 @@ int orc_accept(int pred, float d2, float rc, int64_t i, int64_t j) {
 }
 
 
+/* ------------------------------------------------------------------------- */
+/* Ray-hit models of the Sphere and Triangle programs.  Both OptiX variants find
+ * pairs by casting short rays at per-particle geometry; a pair is added once per
+ * any-hit call that passes the program's filter, so the number of times pair (i, j)
+ * enters particle i's energy (its multiplicity) follows from the geometry.  These are
+ * literal restatements: every ray of particle i is intersected with every primitive of
+ * particle j in FP64, and the any-hit program's filter is applied to each hit.
+ * What OptiX itself decides is closed driver code; the model assumes that a ray
+ * reports a sphere once (at its first surface crossing with t in [tmin, tmax]) and a
+ * two-triangle panel once (the two triangles share a diagonal, and the intersector is
+ * watertight without double hits).  Hits within FP rounding of a primitive's edge or of
+ * tmin / tmax, and the reference's --use_fast_math square root, are not modelled. */
+
+/* optixMultiSphere: particle j is a sphere of radius R = c*0.81649658092f + 1e-5f
+ * (SDK/optixMultiSphere/optixSphere.cpp:156-157, FP32).  Particle i casts three axis
+ * rays from p_i - h e_a, h = c*0.81649658092f, direction e_a, t in [0, 2h]
+ * (optixLJ.cu:77-128).  The any-hit program (optixLJ.cu:150-194) adds the pair when
+ * d2 < c^2 and the ray's axis is the one closest to p_j, with ties going to x, then y,
+ * then z: so each ray of i decides for the pairs nearest its own axis.  d2 is the
+ * canonical squared distance here (the reference's sum of squares may be contracted
+ * by the compiler). */
+static int sphere_mult(float c, const float pi[3], const float pj[3]) {
+    const float h = c * 0.81649658092f;
+    const float rad = c * 0.81649658092f + 1e-5f;
+    const double tmax = 2.0 * (double)h;
+    int m = 0;
+    for (int a = 0; a < 3; ++a) {
+        float o[3];
+        for (int b = 0; b < 3; ++b) o[b] = pi[b] - h * (b == a ? 1.0f : 0.0f);
+        /* |o + t e_a - p_j|^2 = R^2 */
+        const double along = (double)pj[a] - (double)o[a];
+        double perp2 = 0.0;
+        for (int b = 0; b < 3; ++b)
+            if (b != a) {
+                const double d = (double)pj[b] - (double)o[b];
+                perp2 += d * d;
+            }
+        const double disc = (double)rad * (double)rad - perp2;
+        if (disc < 0.0) continue;
+        const double sq = sqrt(disc), t1 = along - sq, t2 = along + sq;
+        const double t = t1 >= 0.0 ? t1 : t2; /* first crossing at or after tmin = 0 */
+        if (!(t >= 0.0 && t <= tmax)) continue;
+        /* any-hit body */
+        const float d2 = orc_d2(pi[0], pi[1], pi[2], pj[0], pj[1], pj[2]);
+        if (!(d2 < c * c)) continue;
+        const float ax = fabsf(pi[0] - pj[0]), ay = fabsf(pi[1] - pj[1]), az = fabsf(pi[2] - pj[2]);
+        const float sx = ax * ax, sy = ay * ay, sz = az * az;
+        const float dax = sy + sz, day = sx + sz, daz = sx + sy;
+        const float closest = fminf(dax, fminf(day, daz));
+        const int mine = (closest == dax && a == 0) || (closest == day && a == 1 && closest != dax) ||
+                         (closest == daz && a == 2 && closest != dax && closest != day);
+        if (mine) ++m;
+    }
+    return m;
+}
+
+/* Moller & Trumbore, "Fast, minimum storage ray-triangle intersection" (1997), in
+ * FP64, for a ray with direction (1, 0, 0); edges and vertices count as inside. */
+static int ray_triangle(const float o[3], const float v0[3], const float v1[3], const float v2[3], double tmin,
+                        double tmax) {
+    const double e1[3] = {(double)v1[0] - v0[0], (double)v1[1] - v0[1], (double)v1[2] - v0[2]};
+    const double e2[3] = {(double)v2[0] - v0[0], (double)v2[1] - v0[1], (double)v2[2] - v0[2]};
+    /* pvec = dir x e2 with dir = (1, 0, 0) */
+    const double pv[3] = {0.0, -e2[2], e2[1]};
+    const double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+    if (det == 0.0) return 0;
+    const double tv[3] = {(double)o[0] - v0[0], (double)o[1] - v0[1], (double)o[2] - v0[2]};
+    const double u = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) / det;
+    if (u < 0.0 || u > 1.0) return 0;
+    const double qv[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+    const double v = qv[0] / det; /* dir . qvec */
+    if (v < 0.0 || u + v > 1.0) return 0;
+    const double t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) / det;
+    return t >= tmin && t <= tmax;
+}
+
+/* optixMultiTriangle: particle j carries two square panels normal to x, at x_j - c/2
+ * and x_j + c/2, each two triangles, whose corners are built in FP32 on the host
+ * (SDK/optixMultiTriangle/optixTriangle.cpp:184-222): y and z span p_j +- (c/2 +
+ * 1e-5).  Particle i casts four rays along +x from (x_i - c/2, y_i +- c/2, z_i +- c/2),
+ * t in [FLT_EPSILON, c + c 1e-5] (optixLJ.cu:82-131).  The any-hit program
+ * (optixLJ.cu:164-217) rebuilds the centre q of the hit panel from the triangle's
+ * vertices, keeps the hit when 1e-4 < |q - p_i| < c, and adds it for ray 0 always and
+ * for rays 1-3 when q differs from p_i in both y and z or lies on the ray's side.
+ * Pairs that sit within 1e-5 of each other in y or z are hit by two rays (four if both),
+ * and pairs within c 1e-5 in x by both panels, so the multiplicity can exceed 1. */
+static int triangle_mult(float c, const float pi[3], const float pj[3]) {
+    const float eps = 0.00001f;
+    const float hc = c / 2;
+    const float lo_off = -hc - eps, hi_off = hc + eps;
+    float corner[8][3];
+    for (int k = 0; k < 8; ++k) {
+        corner[k][2] = pj[2] + ((k & 1) ? hi_off : lo_off);
+        corner[k][1] = pj[1] + ((k & 2) ? hi_off : lo_off);
+        corner[k][0] = pj[0] + ((k & 4) ? hc : -hc);
+    }
+    static const int tri[4][3] = {{0, 1, 3}, {0, 2, 3}, {4, 5, 7}, {4, 6, 7}};
+    const float half = c / 2;
+    const double tmin = (double)1.19209290e-07F;
+    const double tmax = (double)(float)((double)(2 * half) + (double)(2 * half) * 0.00001);
+    int m = 0;
+    for (int r = 0; r < 4; ++r) {
+        const float ycoef = (r & 1) ? -1.0f : 1.0f, zcoef = (r & 2) ? -1.0f : 1.0f;
+        const float o[3] = {pi[0] - half, pi[1] + half * ycoef, pi[2] + half * zcoef};
+        int seen[2] = {0, 0};
+        for (int t = 0; t < 4; ++t) {
+            const int panel = t / 2;
+            if (seen[panel]) continue;
+            const float* v0 = corner[tri[t][0]];
+            const float* v1 = corner[tri[t][1]];
+            const float* v2 = corner[tri[t][2]];
+            if (!ray_triangle(o, v0, v1, v2, tmin, tmax)) continue;
+            seen[panel] = 1;
+            /* any-hit body */
+            float q[3];
+            q[1] = (fmaxf(v0[1], fmaxf(v1[1], v2[1])) + fminf(v0[1], fminf(v1[1], v2[1]))) / 2;
+            q[2] = (fmaxf(v0[2], fmaxf(v1[2], v2[2])) + fminf(v0[2], fminf(v1[2], v2[2]))) / 2;
+            q[0] = panel == 0 ? v0[0] + c / 2 : v0[0] - c / 2;
+            const float d = sqrtf(orc_d2(pi[0], pi[1], pi[2], q[0], q[1], q[2]));
+            if (!(d < c && (double)d > 0.0001)) continue;
+            const int ok = (pi[1] != q[1] && pi[2] != q[2]) || ((pi[2] < q[2] && r == 0) || (pi[2] > q[2] && r == 2)) ||
+                           ((pi[1] < q[1] && r == 0) || (pi[1] > q[1] && r == 1)) || r == 0;
+            if (ok) ++m;
+        }
+    }
+    return m;
+}
+
+int orc_pair_mult(const orc_params* p, float xi, float yi, float zi, float xj, float yj, float zj, int64_t i,
+                  int64_t j) {
+    const float pi[3] = {xi, yi, zi}, pj[3] = {xj, yj, zj};
+    if (i == j) return 0;
+    switch (p->predicate) {
+    case ORC_PRED_SPHERE: return sphere_mult(p->rc, pi, pj);
+    case ORC_PRED_TRIANGLE: return triangle_mult(p->rc, pi, pj);
+    default: return orc_accept(p->predicate, orc_d2(xi, yi, zi, xj, yj, zj), p->rc, i, j);
+    }
+}
+
 uint64_t orc_pairs_brute(const orc_params* p, const float* x, const float* y, const float* z,
                          int64_t n, uint64_t* row_offsets, uint32_t* cols, uint64_t cap) {
     uint64_t m = 0;
@@ -207,8 +346,7 @@ uint64_t orc_pairs_brute(const orc_params* p, const float* x, const float* y, co
         if (row_offsets) row_offsets[i] = m;
         for (int64_t j = 0; j < n; ++j) {
             if (j == i) continue;
-            const float d2 = orc_d2(x[i], y[i], z[i], x[j], y[j], z[j]);
-            if (orc_accept(p->predicate, d2, p->rc, i, j)) {
+            if (orc_pair_mult(p, x[i], y[i], z[i], x[j], y[j], z[j], i, j) > 0) {
                 if (cols && m < cap) cols[m] = (uint32_t)j;
                 ++m;
             }
@@ -272,8 +410,7 @@ uint64_t orc_pairs_cellgrid(const orc_params* p, const float* x, const float* y,
                     for (uint32_t s = g.offsets[c]; s < g.offsets[c + 1]; ++s) {
                         const uint32_t j = g.perm[s];
                         if ((int64_t)j == i) continue;
-                        const float d2 = orc_d2(x[i], y[i], z[i], x[j], y[j], z[j]);
-                        if (orc_accept(p->predicate, d2, p->rc, i, j)) {
+                        if (orc_pair_mult(p, x[i], y[i], z[i], x[j], y[j], z[j], i, j) > 0) {
                             if (cols && m < cap) cols[m] = j;
                             ++m;
                         }
@@ -284,6 +421,65 @@ uint64_t orc_pairs_cellgrid(const orc_params* p, const float* x, const float* y,
         if (cols && m <= cap) qsort(cols + row_start, (size_t)(m - row_start), sizeof(uint32_t), cmp_u32);
     }
     if (row_offsets) row_offsets[n] = m;
+    grid_free(&g);
+    return m;
+}
+
+/* The rows of orc_pairs_cellgrid for a subset of particles only (row r of the
+ * output is particle subset[r]), OpenMP over rows: for checking a few thousand rows
+ * of a large system.  Two passes: counts, then the sorted rows. */
+static uint64_t row_fill(const orc_params* p, const grid_ctx* g, const float* x, const float* y, const float* z,
+                         int64_t i, uint32_t* out) {
+    const int32_t gx = g->dims[0], gy = g->dims[1], gz = g->dims[2];
+    const uint32_t key = g->keys[i];
+    const int32_t cx = (int32_t)(key % (uint32_t)gx);
+    const int32_t cy = (int32_t)((key / (uint32_t)gx) % (uint32_t)gy);
+    const int32_t cz = (int32_t)(key / ((uint32_t)gx * (uint32_t)gy));
+    uint64_t m = 0;
+    for (int32_t iz = cz - 1; iz <= cz + 1; ++iz) {
+        if (iz < 0 || iz >= gz) continue;
+        for (int32_t iy = cy - 1; iy <= cy + 1; ++iy) {
+            if (iy < 0 || iy >= gy) continue;
+            for (int32_t ix = cx - 1; ix <= cx + 1; ++ix) {
+                if (ix < 0 || ix >= gx) continue;
+                const int64_t c = ((int64_t)iz * gy + iy) * gx + ix;
+                for (uint32_t s = g->offsets[c]; s < g->offsets[c + 1]; ++s) {
+                    const uint32_t j = g->perm[s];
+                    if ((int64_t)j == i) continue;
+                    if (orc_pair_mult(p, x[i], y[i], z[i], x[j], y[j], z[j], i, j) > 0) {
+                        if (out) out[m] = j;
+                        ++m;
+                    }
+                }
+            }
+        }
+    }
+    if (out) qsort(out, (size_t)m, sizeof(uint32_t), cmp_u32);
+    return m;
+}
+
+uint64_t orc_pairs_rows(const orc_params* p, const float* x, const float* y, const float* z, int64_t n,
+                        const int64_t* subset, int64_t nsub, uint64_t* row_offsets, uint32_t* cols, uint64_t cap,
+                        int nthreads) {
+    grid_ctx g;
+    if (grid_build(p, x, y, z, n, &g)) return 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    uint64_t* cnt = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(nsub > 0 ? nsub : 1));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t r = 0; r < nsub; ++r) cnt[r] = row_fill(p, &g, x, y, z, subset[r], NULL);
+    uint64_t m = 0;
+    for (int64_t r = 0; r < nsub; ++r) {
+        row_offsets[r] = m;
+        m += cnt[r];
+    }
+    row_offsets[nsub] = m;
+    if (cols && m <= cap) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t r = 0; r < nsub; ++r) row_fill(p, &g, x, y, z, subset[r], cols + row_offsets[r]);
+    }
+    free(cnt);
     grid_free(&g);
     return m;
 }
@@ -335,21 +531,21 @@ int orc_forces_f64(const orc_params* p, const float* x, const float* y, const fl
                         const uint32_t j = g.perm[s];
                         if ((int64_t)j == i) continue;
                         ++nc_i;
-                        const float d2f = orc_d2(x[i], y[i], z[i], x[j], y[j], z[j]);
-                        if (!orc_accept(p->predicate, d2f, p->rc, i, j)) continue;
-                        ++np_i;
+                        const int mult = orc_pair_mult(p, x[i], y[i], z[i], x[j], y[j], z[j], i, j);
+                        if (!mult) continue;
+                        np_i += (uint64_t)mult;
                         const double dx = (double)x[j] - (double)x[i];
                         const double dy = (double)y[j] - (double)y[i];
                         const double dz = (double)z[j] - (double)z[i];
                         const double r2 = dx * dx + dy * dy + dz * dz + soft;
                         const double s2 = sig2 / r2;
                         const double s6 = s2 * s2 * s2;
-                        const double w = 24.0 * eps * s6 * (2.0 * s6 - 1.0) / r2;
+                        const double w = mult * (24.0 * eps * s6 * (2.0 * s6 - 1.0) / r2);
                         ax -= w * dx;
                         ay -= w * dy;
                         az -= w * dz;
                         ac += fabs(w) * sqrt(dx * dx + dy * dy + dz * dz);
-                        ae += 4.0 * eps * s6 * (s6 - 1.0);
+                        ae += mult * (4.0 * eps * s6 * (s6 - 1.0));
                         av += w * r2;
                     }
                 }
@@ -438,17 +634,22 @@ int orc_forces_f32_cellgrid(const orc_params* p, const float* x, const float* y,
                     if ((int64_t)t == s) continue;
                     const float dx = sx[t] - xi, dy = sy[t] - yi, dz = sz[t] - zi;
                     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    if (!orc_accept(p->predicate, d2, p->rc, s, t)) continue;
-                    ++np_i;
+                    int mult;
+                    if (p->predicate == ORC_PRED_SPHERE || p->predicate == ORC_PRED_TRIANGLE)
+                        mult = orc_pair_mult(p, xi, yi, zi, sx[t], sy[t], sz[t], s, t);
+                    else
+                        mult = orc_accept(p->predicate, d2, p->rc, s, t);
+                    if (!mult) continue;
+                    np_i += (uint64_t)mult;
                     const float r2 = d2 + soft;
                     const float ir2 = 1.0f / r2;
                     const float s2 = sig2 * ir2;
                     const float s6 = s2 * s2 * s2;
-                    const float w = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;
+                    const float w = (float)mult * (e24 * s6 * (2.0f * s6 - 1.0f) * ir2);
                     ax -= w * dx;
                     ay -= w * dy;
                     az -= w * dz;
-                    ae += e4 * s6 * (s6 - 1.0f);
+                    ae += (float)mult * (e4 * s6 * (s6 - 1.0f));
                     av += w * r2;
                 }
             }

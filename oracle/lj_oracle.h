@@ -15,12 +15,18 @@
  *   orc_pairs_brute / orc_pairs_cellgrid
  *                                   each other (O(N^2) vs 27-cell), exact-rational
  *                                   d^2 on small sets, symmetry
+ *   orc_pairs_rows                  the matching rows of orc_pairs_cellgrid
  *   orc_forces_f64                  two-body closed forms (V(sigma)=0, F(sigma)=24eps/sigma,
  *                                   minimum at 2^(1/6) sigma), simple-cubic lattice sums,
  *                                   Newton's third law, finite-difference gradient,
  *                                   permutation and translation invariance
  *   orc_forces_f32_cellgrid         orc_forces_f64 within the FP32 error bound
  *   orc_cudalj_energy               numpy FP64 restatement of cudaLJ.cu:28-56 on tiny sets
+ *   orc_pair_mult (SPHERE, TRIANGLE) hand-placed pairs whose multiplicity follows from
+ *                                   the geometry (generic 1, coincident-axis 2 and 4,
+ *                                   panel-overlap 2, near-coincident misses), an
+ *                                   independent numpy ray caster, and the distance
+ *                                   predicates it reduces to on generic pairs
  * The reference itself cannot be compiled here (it needs nvcc, cuRAND and the
  * OptiX driver), so there is no oracle/_ref build.
  */
@@ -37,8 +43,8 @@ extern "C" {
 #define ORC_PRED_AABB     0 /* 1e-16 <= d2 <= rc2      SDK/optixMultiSphereAABB/optixLJ.cu:162 */
 #define ORC_PRED_CUSTOM   1 /* 1e-16 <= d2 <  rc2      SDK/optixMultiCustom/optixMultiCustomAABB.cu:59 */
 #define ORC_PRED_CUDALJ   2 /* every 27-cell candidate j != i, softened, Cuda-version/cudaLJ.cu:319-337 */
-#define ORC_PRED_SPHERE   3 /* 0 < d2 < rc2            SDK/optixMultiSphere/optixLJ.cu:166 */
-#define ORC_PRED_TRIANGLE 4 /* 1e-4 < sqrt(d2) < rc    SDK/optixMultiTriangle/optixLJ.cu:188 */
+#define ORC_PRED_SPHERE   3 /* ray-hit model: d2 < rc2, closest-axis ray hits  SDK/optixMultiSphere/optixLJ.cu:77-194 */
+#define ORC_PRED_TRIANGLE 4 /* ray-hit model, multiplicity  SDK/optixMultiTriangle/optixLJ.cu:82-217 */
 
 typedef struct orc_params {
     float rc, sigma, epsilon, softening2;
@@ -70,8 +76,17 @@ uint64_t orc_candidates_direct(const orc_params* p, const float* x, const float*
 
 /* Canonical FP32 squared distance, d2 = fma(dz,dz, fma(dy,dy, dx*dx)), d = pj - pi. */
 float orc_d2(float xi, float yi, float zi, float xj, float yj, float zj);
-/* Predicate on one pair (for CUDALJ: i != j only). */
+/* Predicate on one pair from its squared distance (for CUDALJ: i != j only; for
+ * SPHERE and TRIANGLE only the distance part of the pair rule, see orc_pair_mult). */
 int orc_accept(int pred, float d2, float rc, int64_t i, int64_t j);
+/* How many times pair (i, j) enters particle i's sums: 0 or 1 for the distance
+ * predicates; for SPHERE and TRIANGLE the number of any-hit calls of the reference's
+ * ray programs that add the pair (ray-hit models of SDK/optixMultiSphere/optixLJ.cu:77-194
+ * and SDK/optixMultiTriangle/optixLJ.cu:82-217; see lj_oracle.c).  The pair lists
+ * hold the pairs with a multiplicity above 0; forces, energies and n_pairs count each
+ * pair as many times as its multiplicity. */
+int orc_pair_mult(const orc_params* p, float xi, float yi, float zi, float xj, float yj, float zj, int64_t i,
+                  int64_t j);
 
 /* O(N^2) neighbour lists in CSR form.  rows ordered by original index i, each row
  * sorted ascending by j.  Returns the number of ordered pairs; writes at most cap
@@ -81,6 +96,13 @@ uint64_t orc_pairs_brute(const orc_params* p, const float* x, const float* y, co
 /* Same lists from the 27-cell grid (candidate enumeration + predicate). */
 uint64_t orc_pairs_cellgrid(const orc_params* p, const float* x, const float* y, const float* z,
                             int64_t n, uint64_t* row_offsets, uint32_t* cols, uint64_t cap);
+
+/* The rows of orc_pairs_cellgrid for particles subset[0..nsub) only: row r of the CSR
+ * output (row_offsets[nsub+1], cols) is particle subset[r], sorted ascending.  OpenMP
+ * over rows; cols is written only when the total fits cap.  Returns the total. */
+uint64_t orc_pairs_rows(const orc_params* p, const float* x, const float* y, const float* z, int64_t n,
+                        const int64_t* subset, int64_t nsub, uint64_t* row_offsets, uint32_t* cols, uint64_t cap,
+                        int nthreads);
 
 /* FP64 forces and energies on the FP32 inputs.  The pair set is decided by the
  * FP32 canonical predicate (so it is the product's pair set); the arithmetic of

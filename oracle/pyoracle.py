@@ -58,10 +58,13 @@ def lib():
         L.orc_d2.argtypes = [ctypes.c_float] * 6
         L.orc_d2.restype = ctypes.c_float
         L.orc_accept.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, i64, i64]
+        L.orc_pair_mult.argtypes = [P] + [ctypes.c_float] * 6 + [i64, i64]
         L.orc_pairs_brute.argtypes = [P, vp, vp, vp, i64, vp, vp, u64]
         L.orc_pairs_brute.restype = u64
         L.orc_pairs_cellgrid.argtypes = [P, vp, vp, vp, i64, vp, vp, u64]
         L.orc_pairs_cellgrid.restype = u64
+        L.orc_pairs_rows.argtypes = [P, vp, vp, vp, i64, vp, i64, vp, vp, u64, ctypes.c_int]
+        L.orc_pairs_rows.restype = u64
         L.orc_forces_f64.argtypes = [P, vp, vp, vp, i64, vp, i64, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int]
         L.orc_forces_f32_cellgrid.argtypes = [P, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, ctypes.c_int]
         L.orc_cudalj_energy.argtypes = [P, vp, vp, vp, i64, vp]
@@ -160,6 +163,18 @@ def pairs_brute(p, x, y, z):
 
 def pairs_cellgrid(p, x, y, z):
     return _pairs(lib().orc_pairs_cellgrid, p, x, y, z)
+
+
+def pairs_rows(p, x, y, z, subset, nthreads=0):
+    """CSR rows (sorted) of the particles in subset, as orc_pairs_cellgrid gives them."""
+    x, y, z = _f32(x), _f32(y), _f32(z)
+    sub = np.ascontiguousarray(subset, np.int64)
+    rows = np.zeros(sub.size + 1, np.uint64)
+    args = (ctypes.byref(p), x.ctypes.data, y.ctypes.data, z.ctypes.data, x.size, sub.ctypes.data, sub.size)
+    m = lib().orc_pairs_rows(*args, rows.ctypes.data, None, 0, nthreads)
+    cols = np.zeros(max(int(m), 1), np.uint32)
+    lib().orc_pairs_rows(*args, rows.ctypes.data, cols.ctypes.data, m, nthreads)
+    return rows, cols[: int(m)]
 
 
 def forces_f64(p, x, y, z, subset=None, nthreads=0):

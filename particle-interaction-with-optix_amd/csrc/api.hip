@@ -1,6 +1,7 @@
 // api.hip -- C ABI entry points: context, validation, binning and reordering.
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -9,6 +10,23 @@
 namespace lj {
 
 static thread_local std::string g_create_err;
+
+std::atomic<long long> g_alloc_budget{-1};
+
+// Every device buffer a context owns (the workspace).
+template <class F>
+void for_each_buffer(lj_context* ctx, F&& f) {
+    BinWork& b = ctx->bin;
+    for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.hist, &b.scan_tmp, &b.gaps, &b.offsets, &b.pos, &b.pos_alt,
+                      &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets, &b.cand})
+        f(*d);
+    LbvhWork& l = ctx->lbvh;
+    for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags}) f(*d);
+    BatchWork& t = ctx->batch;
+    for (DevBuf* d : {&t.sys, &t.hints, &t.npair, &t.ev, &t.wrec}) f(*d);
+    f(ctx->tune_out);
+}
+
 
 int fail(lj_context* ctx, int code, const char* fmt, ...) {
     char buf[512];
@@ -91,6 +109,15 @@ int make_geom(lj_context* ctx, const lj_params* p, int64_t n, GridGeom* g, PairC
     if (p->depth != 0 && p->depth != 1 && p->depth != 8 && p->depth != 16 && p->depth != 32)
         return fail(ctx, LJ_ERR_INVALID_ARG, "depth must be 0, 1, 8, 16 or 32 (got %d)", (int)p->depth);
     c->tri_lo = tri_lo();
+    // ray-model constants, rounded as the reference's host and raygen code rounds them
+    // (SDK/optixMultiSphere/optixLJ.cu:95, optixSphere.cpp:157;
+    // SDK/optixMultiTriangle/optixTriangle.cpp:184-205, optixLJ.cu:94-127)
+    c->sph_h = p->rc * 0.81649658092f;
+    c->sph_rad = p->rc * 0.81649658092f + 1e-5f;
+    c->tri_hc = p->rc / 2;
+    c->tri_ext = c->tri_hc + 0.00001f;
+    c->tri_tmin = (double)1.19209290e-07F;
+    c->tri_tmax = (double)(float)((double)(2 * c->tri_hc) + (double)(2 * c->tri_hc) * 0.00001);
     c->e4 = 4.0 * (double)p->epsilon;
     c->e24 = 24.0 * (double)p->epsilon;
     c->pred = p->predicate;
@@ -171,6 +198,9 @@ int lj_context_create(lj_context** out, int device) {
     if (!ctx) return fail(nullptr, LJ_ERR_OOM, "host allocation failed");
     ctx->device = device;
     ctx->num_cus = prop.multiProcessorCount;
+    // test hook: LJ_TEST_FAIL_ALLOC_AFTER=k lets k more device allocations succeed
+    const char* inj = std::getenv("LJ_TEST_FAIL_ALLOC_AFTER");
+    g_alloc_budget.store(inj && *inj ? std::strtoll(inj, nullptr, 10) : -1, std::memory_order_relaxed);
     *out = ctx;
     return LJ_OK;
 }
@@ -181,16 +211,7 @@ static void release_all(lj_context* ctx) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
-    BinWork& b = ctx->bin;
-    for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.hist, &b.scan_tmp, &b.gaps, &b.offsets, &b.pos, &b.pos_alt,
-                      &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets, &b.cand})
-        d->release();
-    LbvhWork& l = ctx->lbvh;
-    for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags})
-        d->release();
-    BatchWork& t = ctx->batch;
-    for (DevBuf* d : {&t.sys, &t.hints, &t.npair, &t.ev, &t.wrec}) d->release();
-    ctx->tune_out.release();
+    for_each_buffer(ctx, [](DevBuf& d) { d.release(); });
 }
 
 int lj_context_destroy(lj_context* ctx) {
@@ -198,6 +219,14 @@ int lj_context_destroy(lj_context* ctx) {
     (void)hipSetDevice(ctx->device);
     release_all(ctx);
     delete ctx;
+    return LJ_OK;
+}
+
+int lj_workspace_generation(lj_context* ctx, uint64_t* generation) {
+    if (!ctx || !generation) return LJ_ERR_INVALID_ARG;
+    uint64_t g = 0;
+    for_each_buffer(ctx, [&](DevBuf& d) { g += d.reallocs; });
+    *generation = g;
     return LJ_OK;
 }
 

@@ -92,7 +92,8 @@ __global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ par
 }
 
 // ------------------------------------------------------------------ pair lists
-// Pass 1: number of accepted partners of each particle, stored by original index.
+// Pass 1: number of accepted partners of each particle, stored by original index (a
+// partner with a multiplicity above one is listed once).
 template <int PRED>
 __global__ __launch_bounds__(256) void k_pair_count(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
                                                     const uint32_t* __restrict__ off, uint32_t n, GridGeom g,
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(256) void k_pair_count(const float4* __restrict__ p
             for (uint32_t t = off[row + x0]; t < off[row + x1 + 1]; ++t) {
                 const float4 pj = pos[t];
                 const float d2 = dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                if (t != k && accept<PRED>(c, d2)) ++cnt;
+                if (t != k && pair_mult<PRED>(c, pi, pj, d2) > 0) ++cnt;
             }
         }
     }
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void k_pair_fill(const float4* __restrict__ po
             for (uint32_t t = off[row + x0]; t < off[row + x1 + 1]; ++t) {
                 const float4 pj = pos[t];
                 const float d2 = dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                if (t != k && accept<PRED>(c, d2)) cols[w++] = __float_as_uint(pj.w);
+                if (t != k && pair_mult<PRED>(c, pi, pj, d2) > 0) cols[w++] = __float_as_uint(pj.w);
             }
         }
     }

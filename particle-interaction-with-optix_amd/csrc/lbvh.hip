@@ -14,7 +14,7 @@
 //     Octrees, and k-d Trees", HPG 2012 (one thread per internal node);
 //  3. node bounds without synchronisation: an internal node covers a contiguous range
 //     of leaves (Morton order), so its bounds are the min/max over that range, read
-//     from six levels of aligned chunk bounds (16, 256, ... 2^24 leaves) plus the
+//     from twelve levels of aligned chunk bounds (4, 16, ..., 2^24 leaves) plus the
 //     leaves at the range's ends.  (Merging child boxes bottom-up would need an
 //     acquire/release handshake per node across workgroups; at device scope on a
 //     multi-XCD part each one writes back and invalidates the L2.)
@@ -29,6 +29,8 @@
 // half-width rc' = rc (1 + 2^-18) against them is the point-in-AABB test of the
 // reference with the rounding of p +- rc moved to the safe side: every pair the
 // predicate can accept has |d_axis| <= rc (1 + 2^-20).
+#include <cmath>
+
 #include "lj_internal.h"
 #include "lj_pair.h"
 #include "lj_sort.h"
@@ -153,9 +155,9 @@ struct Bounds {
     }
 };
 
-// Bounds of aligned chunks: chunk c of the output covers inputs [16 c, 16 c + 16)
-// (leaf positions when `leaves`, else (lo, hi) pairs of the level below); 16 lanes per
-// chunk.
+// Bounds of aligned chunks: chunk c of the output covers inputs [kChunk c, kChunk c +
+// kChunk) (leaf positions when `leaves`, else (lo, hi) pairs of the level below);
+// kChunk lanes per chunk.
 __global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__ in, bool leaves, uint32_t count,
                                                       uint32_t nchunks, float4* __restrict__ out) {
     const uint32_t e = blockIdx.x * 256u + threadIdx.x;  // input element
@@ -242,8 +244,8 @@ __device__ inline bool overlaps(float4 p, float h, float4 lo, float4 hi) {
 // scan the listed leaves in list order; repeat while the walk is unfinished (it stops
 // at a node and resumes there, having no stack).  Interleaving the two made every wave
 // step pay for a bucket scan as soon as one lane needed one.  The leaves are visited
-// in walk order either way.  Returns false if the walk exceeds 2n steps (impossible
-// for a well-formed tree, but checked).  list: this lane's entries, at list[q * stride].
+// in walk order either way.  Returns false if the walk exceeds `limit` steps (2n: a
+// well-formed tree never needs more, but it is checked).  list: this lane's entries, at list[q * stride].
 // The list is LDS (kList * kLbvhBlock ints per block), which bounds the waves per CU:
 // 16 entries and 64-thread blocks measured fastest (list 8..64, blocks 64..256).
 constexpr int kList = 16;
@@ -254,8 +256,8 @@ static_assert(kLeafGroup - 1 <= kPosSpare, "unclamped leaf loads need spare entr
 template <class F>
 __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                 const float4* __restrict__ box, const int* __restrict__ rope_by_end, int n, int k,
-                                float4 p, float h, unsigned long long& tests, int* __restrict__ list, uint32_t stride,
-                                F&& f) {
+                                float4 p, float h, uint32_t limit, unsigned long long& tests, int* __restrict__ list,
+                                uint32_t stride, F&& f) {
     if (n == 1) return true;
     auto leaf_test = [&](int slot, const float4& pj) {
         if ((pj.x - p.x) <= h && (p.x - pj.x) <= h && (pj.y - p.y) <= h && (p.y - pj.y) <= h &&
@@ -265,7 +267,6 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
         }
     };
     int node = n <= kBucket ? (0 | kCollapsed) : 0;  // a small tree is one bucket
-    const uint32_t limit = 2u * (uint32_t)n;
     uint32_t steps = 0;
     bool ok = true;
     while (node != kSentinel) {
@@ -315,7 +316,7 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
 
 template <int PRED, int PREC, bool VIR>
 __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restrict__ leaf, const int2* __restrict__ range,
-                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
+                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, uint32_t limit, PairConst c,
                                                     float* __restrict__ fx, float* __restrict__ fy,
                                                     float* __restrict__ fz, float* __restrict__ e,
                                                     double* __restrict__ partials, lj_result* __restrict__ res,
@@ -332,12 +333,13 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
         const float4 pi = leaf[k];
         acc.set_i(pi);
         acc.allow_exact_diff(c, pi, true);  // over the lanes with a query
-        const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, list, blockDim.x, [&](int, float4 pj) {
+        const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x, [&](int, float4 pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float d2 = dist2(dx, dy, dz);
-            if (!accept<PRED>(c, d2)) return;
-            ++np;
-            acc.add(c, pj, dx, dy, dz, d2);
+            const int mult = pair_mult<PRED>(c, pi, pj, d2);
+            if (!mult) return;
+            np += (unsigned long long)mult;
+            acc.add(c, pj, dx, dy, dz, d2, mult);
         });
         if (!ok) atomicOr(overflow, 1);
         const uint32_t i = __float_as_uint(pi.w);
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
 
 template <int PRED>
 __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_count(const float4* __restrict__ leaf, const int2* __restrict__ range,
-                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
+                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, uint32_t limit, PairConst c,
                                                     uint32_t* __restrict__ rowcnt, int* __restrict__ overflow) {
     extern __shared__ int lds_list[];
     int* list = lds_list + threadIdx.x;
@@ -371,8 +373,8 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_count(const float4* __restr
     const float4 pi = leaf[k];
     uint32_t cnt = 0;
     unsigned long long tests = 0;
-    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, list, blockDim.x, [&](int, float4 pj) {
-        if (accept<PRED>(c, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z))) ++cnt;
+    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x, [&](int, float4 pj) {
+        if (pair_mult<PRED>(c, pi, pj, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z)) > 0) ++cnt;
     });
     if (!ok) atomicOr(overflow, 1);
     rowcnt[__float_as_uint(pi.w)] = cnt;
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_count(const float4* __restr
 
 template <int PRED>
 __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_fill(const float4* __restrict__ leaf, const int2* __restrict__ range,
-                                                   const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, PairConst c,
+                                                   const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, uint32_t limit, PairConst c,
                                                    const unsigned long long* __restrict__ rows,
                                                    uint32_t* __restrict__ cols, int* __restrict__ overflow) {
     extern __shared__ int lds_list[];
@@ -392,8 +394,9 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_fill(const float4* __restri
     const unsigned long long r0 = rows[i], r1 = rows[i + 1];
     unsigned long long w = r0;
     unsigned long long tests = 0;
-    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, tests, list, blockDim.x, [&](int, float4 pj) {
-        if (w < r1 && accept<PRED>(c, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z))) cols[w++] = __float_as_uint(pj.w);
+    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x, [&](int, float4 pj) {
+        if (w < r1 && pair_mult<PRED>(c, pi, pj, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z)) > 0)
+            cols[w++] = __float_as_uint(pj.w);
     });
     if (!ok) atomicOr(overflow, 1);
     for (unsigned long long a = r0 + 1; a < w; ++a) {
@@ -407,7 +410,24 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_fill(const float4* __restri
     }
 }
 
+// Marks the results of a force call whose walk tripped its step guard: n_pairs =
+// UINT64_MAX and NaN totals, so a caller that reads the results sees the failure
+// without the call synchronising.
+__global__ void k_poison_on_overflow(const int* __restrict__ overflow, lj_result* __restrict__ res) {
+    if (threadIdx.x == 0 && *overflow) {
+        res->n_pairs = ~0ull;
+        res->e_total = __builtin_nan("");
+        res->virial = __builtin_nan("");
+    }
+}
+
 }  // namespace
+
+// Step guard of every walk: 2n steps, or one step under LJ_FLAG_WALK_GUARD_TEST (the
+// host-side way to exercise the guard's error path; every kernel still ends normally).
+static uint32_t walk_limit(const lj_params* p, int64_t n) {
+    return (p->flags & LJ_FLAG_WALK_GUARD_TEST) ? 1u : 2u * (uint32_t)(n > 0 ? n : 1);
+}
 
 // The traversal's overflow flag.
 static int* overflow_flag(lj_context* ctx, int64_t) { return ctx->lbvh.flags.as<int>(); }
@@ -442,6 +462,7 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     }
     // the sorted payload is the leaf array: positions in Morton order, w = original index
     LJ_TRY(sort_particles(ctx, x, y, z, n, op, 3 * kMortonBits, s));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, overflow_flag(ctx, n), 1u, 0);
     if (n == 0) return LJ_OK;
     const uint32_t* code = B.sorted_keys();
     int2* range = L.range.as<int2>();
@@ -450,7 +471,6 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, s, code, (int)n, L.node_child.as<int2>(),
                            range, rope);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, overflow_flag(ctx, n), 1u, 0);
     // chunk bounds, level by level (level 0 from the leaves, level l + 1 from level l)
     const float4* leaves = B.pos.as<float4>();
     {
@@ -473,7 +493,19 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     return LJ_OK;
 }
 
-static float query_half_width(const lj_params* p) { return p->rc * (1.0f + 0x1p-18f); }
+// Query half-width.  Every pair a cutoff predicate accepts has |d_axis| <= rc (1 +
+// 2^-20); the TRIANGLE ray model measures the cutoff to the centre rebuilt from the
+// panel corners, which can lie a few ulp of the coordinates off the particle, so its
+// boxes grow by 2^-20 of the largest coordinate magnitude of the box.
+static float query_half_width(const lj_params* p) {
+    float h = p->rc * (1.0f + 0x1p-18f);
+    if (p->predicate == LJ_PRED_TRIANGLE) {
+        float mag = 0.0f;
+        for (int d = 0; d < 3; ++d) mag = std::fmax(mag, std::fmax(std::fabs(p->box_lo[d]), std::fabs(p->box_hi[d])));
+        h += mag * 0x1p-20f;
+    }
+    return h;
+}
 
 static int check_lbvh_params(lj_context* ctx, const lj_params* p, int64_t n) {
     if (p->predicate == LJ_PRED_CUDALJ)
@@ -516,13 +548,14 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     double* partials = ctx->bin.partials.as<double>();
     int* ovf = overflow_flag(ctx, n);
     const float h = query_half_width(p);
+    const uint32_t lim = walk_limit(p, n);
     const float4* leaf = ctx->bin.pos.as<float4>();
     const int2* range = ctx->lbvh.range.as<int2>();
     const int* rope = ctx->lbvh.rope.as<int>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
 #define LJ_LB(PRED, PREC, VIR)                                                                                        \
-    hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, c, fx, \
+    hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, fx, \
                        fy, fz, e, partials, res, ovf)
 #define LJ_LB_PREC(PRED)                                                                                    \
     switch (p->precision) {                                                                                 \
@@ -540,11 +573,12 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
         }
     } else {
         hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true>), dim3(1), dim3(kLbvhBlock), lds, s, leaf, range, rope, box,
-                           0, h, c, fx, fy, fz, e, partials, res, ovf);
+                           0, h, lim, c, fx, fy, fz, e, partials, res, ovf);
     }
 #undef LJ_LB_PREC
 #undef LJ_LB
     finalize_energy(partials, nb, res, s);
+    hipLaunchKernelGGL(k_poison_on_overflow, dim3(1), dim3(64), 0, s, (const int*)ovf, res);
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }
@@ -568,13 +602,14 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     uint32_t* rowcnt = ctx->bin.rows.as<uint32_t>();
     int* ovf = overflow_flag(ctx, n);
     const float h = query_half_width(p);
+    const uint32_t lim = walk_limit(p, n);
     const float4* leaf = ctx->bin.pos.as<float4>();
     const int2* range = ctx->lbvh.range.as<int2>();
     const int* rope = ctx->lbvh.rope.as<int>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
     if (n > 0) {
         switch (p->predicate) {
-#define LJ_C(P) case P: hipLaunchKernelGGL(k_lbvh_count<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, c, rowcnt, ovf); break;
+#define LJ_C(P) case P: hipLaunchKernelGGL(k_lbvh_count<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, rowcnt, ovf); break;
             LJ_C(LJ_PRED_AABB) LJ_C(LJ_PRED_CUSTOM) LJ_C(LJ_PRED_SPHERE) LJ_C(LJ_PRED_TRIANGLE)
 #undef LJ_C
         }
@@ -585,12 +620,12 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     LJ_HIP(ctx, hipMemcpyAsync(&total, row_offsets + n, sizeof total, hipMemcpyDeviceToHost, s));
     LJ_HIP(ctx, hipMemcpyAsync(&overflow, ovf, sizeof overflow, hipMemcpyDeviceToHost, s));
     LJ_HIP(ctx, hipStreamSynchronize(s));
-    if (overflow) return fail(ctx, LJ_ERR_OVERFLOW, "LBVH walk exceeded 2n steps: malformed tree (flag %d)", overflow);
+    if (overflow) return fail(ctx, LJ_ERR_OVERFLOW, "LBVH walk exceeded its step guard (flag %d)", overflow);
     if (needed) *needed = total;
     if (total > capacity) return fail(ctx, LJ_ERR_CAPACITY, "pair list needs %llu entries, capacity is %llu", total, capacity);
     if (total == 0) return LJ_OK;
     switch (p->predicate) {
-#define LJ_F(P) case P: hipLaunchKernelGGL(k_lbvh_fill<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, c, (const unsigned long long*)row_offsets, cols, ovf); break;
+#define LJ_F(P) case P: hipLaunchKernelGGL(k_lbvh_fill<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, (const unsigned long long*)row_offsets, cols, ovf); break;
         LJ_F(LJ_PRED_AABB) LJ_F(LJ_PRED_CUSTOM) LJ_F(LJ_PRED_SPHERE) LJ_F(LJ_PRED_TRIANGLE)
 #undef LJ_F
     }

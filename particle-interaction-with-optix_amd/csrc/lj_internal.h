@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -15,13 +16,30 @@ struct lj_context;
 
 namespace lj {
 
+// Allocation-failure injection for tests: when >= 0, the number of device
+// allocations that may still succeed in this process before every later one reports
+// hipErrorOutOfMemory (without calling hipMalloc).  Set by lj_context_create from the
+// environment variable LJ_TEST_FAIL_ALLOC_AFTER; -1 (the default) turns it off.
+extern std::atomic<long long> g_alloc_budget;
+
+inline bool alloc_fault_injected() {
+    long long b = g_alloc_budget.load(std::memory_order_relaxed);
+    while (b > 0 && !g_alloc_budget.compare_exchange_weak(b, b - 1, std::memory_order_relaxed)) {
+    }
+    return b == 0;
+}
+
 // A grow-only device buffer.  grow() reallocates only when the request exceeds the
-// current capacity, so steady-state calls of one size allocate nothing.
+// current capacity, so steady-state calls of one size allocate nothing.  `reallocs`
+// counts the allocations, so a caller holding the buffer's address (a captured
+// graph) can tell that it moved (lj_workspace_generation).
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    uint64_t reallocs = 0;
     hipError_t grow(size_t need) {
         if (need <= bytes) return hipSuccess;
+        if (alloc_fault_injected()) return hipErrorOutOfMemory;
         if (ptr) {
             hipError_t e = hipFree(ptr);
             if (e != hipSuccess) return e;
@@ -29,6 +47,7 @@ struct DevBuf {
             bytes = 0;
         }
         size_t cap = need < 256 ? 256 : need;
+        ++reallocs;
         hipError_t e = hipMalloc(&ptr, cap);
         if (e != hipSuccess) { ptr = nullptr; return e; }
         bytes = cap;

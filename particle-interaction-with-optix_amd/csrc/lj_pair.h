@@ -85,7 +85,9 @@ struct PairSums {
         const bool ok = !active || (!SOFT && fabsf(pi.x) >= reach && fabsf(pi.y) >= reach && fabsf(pi.z) >= reach);
         exact = __all(ok);
     }
-    __device__ inline void add(const PairConst& c, float4 pj, float dx, float dy, float dz, float d2) {
+    // mult: how many times the pair counts (the TRIANGLE ray model can count a pair
+    // more than once); 1 for every other predicate.
+    __device__ inline void add(const PairConst& c, float4 pj, float dx, float dy, float dz, float d2, int mult = 1) {
         if constexpr (PREC == LJ_PREC_MIXED) {
             double ddx, ddy, ddz;
             if (exact) {
@@ -102,8 +104,12 @@ struct PairSums {
             const double ir2 = rcp_f64(r2);
             const double s2 = c.sig2d * ir2;
             const double s6 = s2 * s2 * s2;
-            const double u = __builtin_fma(s6, s6, -s6);  // energy term s6 (s6 - 1)
-            const double t = __builtin_fma(s6, s6, u);    // s6 (2 s6 - 1)
+            double u = __builtin_fma(s6, s6, -s6);  // energy term s6 (s6 - 1)
+            double t = __builtin_fma(s6, s6, u);    // s6 (2 s6 - 1)
+            if (mult != 1) {
+                u *= (double)mult;
+                t *= (double)mult;
+            }
             const double w = t * ir2;
             ax.fma(w, ddx);
             ay.fma(w, ddy);
@@ -116,12 +122,17 @@ struct PairSums {
             const float ir2 = __builtin_amdgcn_rcpf(r2);
             const float s2 = c.sig2 * ir2;
             const float s6 = s2 * s2 * s2;
-            const float t = __builtin_fmaf(s6 + s6, s6, -s6);
+            float t = __builtin_fmaf(s6 + s6, s6, -s6);
+            float u = __builtin_fmaf(s6, s6, -s6);
+            if (mult != 1) {
+                t *= (float)mult;
+                u *= (float)mult;
+            }
             const float w = t * ir2;
             ax.fma(w, dx);
             ay.fma(w, dy);
             az.fma(w, dz);
-            ae.add(__builtin_fmaf(s6, s6, -s6));
+            ae.add(u);
             if constexpr (VIR) av.add(t);
         }
     }

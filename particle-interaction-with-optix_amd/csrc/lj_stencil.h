@@ -107,6 +107,9 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
                                          uint32_t key, int Rx, int R, const Trim& tr, const PairConst& c,
                                          uint32_t* __restrict__ queue, uint32_t stride, unsigned depth,
                                          PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR>& acc) {
+    // only the TRIANGLE ray model counts a pair more than once (queued partners then
+    // get their multiplicity again when flushed)
+    constexpr bool kMultiple = PRED == LJ_PRED_TRIANGLE;
     unsigned pc = 0;          // queued partners of this lane
     uint32_t* tail = queue;   // = queue + pc * stride
     uint32_t np = 0;
@@ -118,11 +121,13 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
             const float4 p1 = pos[a1 ? queue[(q + 1) * stride] : kload];
             if (a0) {
                 const float dx = p0.x - pi.x, dy = p0.y - pi.y, dz = p0.z - pi.z;
-                acc.add(c, p0, dx, dy, dz, dist2(dx, dy, dz));
+                const float d2 = dist2(dx, dy, dz);
+                acc.add(c, p0, dx, dy, dz, d2, kMultiple ? pair_mult<PRED>(c, pi, p0, d2) : 1);
             }
             if (a1) {
                 const float dx = p1.x - pi.x, dy = p1.y - pi.y, dz = p1.z - pi.z;
-                acc.add(c, p1, dx, dy, dz, dist2(dx, dy, dz));
+                const float d2 = dist2(dx, dy, dz);
+                acc.add(c, p1, dx, dy, dz, d2, kMultiple ? pair_mult<PRED>(c, pi, p1, d2) : 1);
             }
         }
         pc = 0;
@@ -136,14 +141,15 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
         if constexpr (PRED == LJ_PRED_CUDALJ) {
             if (t == k) return;
         }
-        if (!accept<PRED>(c, d2)) return;
-        ++np;
+        const int mult = pair_mult<PRED>(c, pi, pj, d2);
+        if (!mult) return;
+        np += (uint32_t)mult;
         if constexpr (QUEUE) {
             *tail = t;
             tail += stride;
             ++pc;
         } else {
-            acc.add(c, pj, dx, dy, dz, d2);
+            acc.add(c, pj, dx, dy, dz, d2, mult);
         }
     };
     int32_t cx = 0, cy = 0, cz = 0;

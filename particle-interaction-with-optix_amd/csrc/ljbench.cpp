@@ -1,10 +1,11 @@
 // ljbench.cpp -- command-line driver with the reference's benchmark sweeps and CSV files.
 //
-//   ljbench cudalj [-gs|--gensurface]
-//       Cuda-version/cudaLJ.cu:701-792: for boxDiv = 2,4,..,32 (surface: ..,128) and
-//       N = boxDiv^3 * 2^k (k = 0..5), NbLoops = 5 runs of executeSimulation (bin + the
-//       27-cell energy kernel with no cutoff, softening 1e-5, sigma = eps = 1);
-//       CSV results[-surface]-YYYYmmdd-HHMMSS.csv with the reference columns.
+//   ljbench cudalj [-gs|--gensurface] [--max-boxdiv M]
+//       Cuda-version/cudaLJ.cu:701-792: for boxDiv = 2,4,..,32 (surface: ..,128; both
+//       capped at M) and N = boxDiv^3 * 2^k (k = 0..5), NbLoops = 5 runs of
+//       executeSimulation (bin + the 27-cell energy kernel with no cutoff, softening
+//       1e-5, sigma = eps = 1); CSV results[-surface]-YYYYmmdd-HHMMSS.csv with the
+//       reference columns.
 //   ljbench aabb [-rb] [-gs] [-us] [-ns N] [-bd D] [--max-boxdiv M]
 //       SDK/optixMultiSphereAABB/optixSphere.cpp:636-830: without -rb one core() call
 //       (N = 10, rc = 1/D) whose first 100 energies are checked against a host brute
@@ -165,7 +166,11 @@ std::tuple<unsigned long long, std::vector<Timing>> execute_simulation(lj_contex
         }
         upload(dev, x, y, z);
         const Timing t = timed(ctx, false, dev, n, p, nullptr);
-        total += read_result(dev).n_candidates;
+        const lj_result r = read_result(dev);
+        total += r.n_candidates;
+        // not printed by the reference: the per-loop counts and the energy total, so a
+        // run can be checked against an independent computation on the same positions
+        std::printf(" - loop %d: candidates %llu e_total %.17g\n", l, r.n_candidates, r.e_total);
         out.push_back(t);
     }
     double ti = 0, tc = 0;
@@ -179,12 +184,12 @@ std::tuple<unsigned long long, std::vector<Timing>> execute_simulation(lj_contex
     return {total, out};
 }
 
-int run_cudalj(bool gensurface) {
+int run_cudalj(bool gensurface, int max_boxdiv) {
     lj_context* ctx = nullptr;
     CHECK_LJ(nullptr, lj_context_create(&ctx, 0));
     Device dev;
     std::vector<Frame> all;
-    const int loops = 5, max_ppc = 32, max_div = gensurface ? 128 : 32;
+    const int loops = 5, max_ppc = 32, max_div = std::min(max_boxdiv, gensurface ? 128 : 32);
     for (int box_div = 2; box_div <= max_div; box_div *= 2) {
         const int nb_boxes = box_div * box_div * box_div;
         for (long long np = nb_boxes; np <= (long long)nb_boxes * max_ppc; np *= 2) {
@@ -280,8 +285,8 @@ std::pair<double, double> core(lj_context* ctx, Device& dev, int n, float radius
                 const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
                 if (d <= radius && d * d >= 1e-16) ref += 4.0 * (std::pow(1.0 / d, 12) - std::pow(1.0 / d, 6));
             }
-            std::printf("Energy for particle %d position %g %g %g is %g it has been computed as %g\n", i, (double)x[i],
-                        (double)y[i], (double)z[i], ref, (double)e[(size_t)i]);
+            std::printf("Energy for particle %d position %.9g %.9g %.9g is %.17g it has been computed as %.9g\n", i,
+                        (double)x[i], (double)y[i], (double)z[i], ref, (double)e[(size_t)i]);
         }
     }
     return {tsort + t.init, t.compute};
@@ -421,7 +426,7 @@ int run_lj(int argc, char** argv) {
 
 void usage(const char* argv0) {
     std::fprintf(stderr,
-                 "Usage: %s cudalj [-gs|--gensurface]\n"
+                 "Usage: %s cudalj [-gs|--gensurface] [--max-boxdiv M]\n"
                  "       %s aabb [-rb|--runbench] [-gs|--gensurface] [-us|--usedsort] [-ns|--nbspheres N]\n"
                  "            [-bd|--boxdivisor D] [--max-boxdiv M]\n"
                  "       %s lj [--n N] [--rc RC] [--gen uniform|surface|clusters] [--loops L] [--refine K]\n"
@@ -454,7 +459,7 @@ int main(int argc, char** argv) {
             usage(argv[0]);
         }
     }
-    if (mode == "cudalj") return run_cudalj(gensurface);
+    if (mode == "cudalj") return run_cudalj(gensurface, max_div);
     if (mode == "aabb") return run_aabb(rb, gensurface, sort, ns, bd, max_div);
     usage(argv[0]);
 }

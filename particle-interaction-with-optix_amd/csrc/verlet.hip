@@ -44,9 +44,10 @@ __global__ __launch_bounds__(256) void k_list_force(const float* __restrict__ x,
             if constexpr (PRED == LJ_PRED_CUDALJ) {
                 if (j == i) continue;
             }
-            if (!accept<PRED>(c, d2)) continue;
-            ++np;
-            acc.add(c, pj, dx, dy, dz, d2);
+            const int mult = pair_mult<PRED>(c, pi, pj, d2);
+            if (!mult) continue;
+            np += (unsigned long long)mult;
+            acc.add(c, pj, dx, dy, dz, d2, mult);
         }
         fx[i] = (float)(-c.e24 * (double)acc.ax.value());
         fy[i] = (float)(-c.e24 * (double)acc.ay.value());

@@ -35,6 +35,7 @@ ERRORS = {
 PRED = {"aabb": 0, "custom": 1, "cudalj": 2, "sphere": 3, "triangle": 4}
 PREC = {"mixed": 0, "fp32": 1, "fp32_comp": 2}
 FLAG_NO_VIRIAL = 1
+FLAG_WALK_GUARD_TEST = 2
 
 GEN = {"uniform": 0, "surface": 1, "clusters": 2}
 
@@ -59,6 +60,7 @@ EXPORTS = (
     "lj_autotune_batched",
     "lj_compute_list",
     "lj_max_displacement",
+    "lj_workspace_generation",
 )
 PHASES = ("bin", "force", "other")
 
@@ -163,6 +165,7 @@ def load(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
         "lj_autotune_batched": ([vp, P(LJSystem), ctypes.c_int32, vp, vp, vp, ctypes.c_int32, P(LJTuning), vp], ctypes.c_int),
         "lj_compute_list": ([vp, vp, vp, vp, i64, P(LJParams), vp, vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "lj_max_displacement": ([vp, vp, vp, vp, vp, vp, vp, i64, vp, vp], ctypes.c_int),
+        "lj_workspace_generation": ([vp, P(u64)], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         if hasattr(lib, name):  # tests check that every declared symbol is exported
@@ -373,6 +376,13 @@ class Engine:
             fn(self.ctx, _ptr(x), _ptr(y), _ptr(z), n, ctypes.byref(p), _ptr(rows), _ptr(cols), capacity, ctypes.byref(needed), self._stream())
         )
         return rows, cols[: needed.value]
+
+    def workspace_generation(self) -> int:
+        """Changes whenever a call reallocates workspace (a captured graph's addresses
+        are then stale): record it after capturing, replay only while it is unchanged."""
+        g = ctypes.c_uint64()
+        self._check(self.lib.lj_workspace_generation(self.ctx, ctypes.byref(g)))
+        return g.value
 
     def profile(self, enable: bool = True):
         """Turns per-phase hipEvent timing on (and resets it) or off."""

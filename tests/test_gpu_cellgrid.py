@@ -103,6 +103,23 @@ FORCE_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7"
                "gap_classes"]
 
 
+@pytest.mark.parametrize("name", ["c1_uniform_100k", "c2_surface_200k", "edges", "single", "empty"])
+def test_reorder_matches_oracle(engine, name):
+    """lj_reorder (the -us pre-sort of SDK/optixMultiSphereAABB/sort.cu:28-41): the
+    positions come back in the oracle's stable cell order, xo = x[perm], with perm the
+    oracle's permutation, bit for bit."""
+    (x, y, z), p = _case(name)
+    n = x.size
+    print(f"[reorder] {name} n={n}", flush=True)
+    _, perm_ref, _ = po.bin(po.from_lj(p), x, y, z)
+    dx, dy, dz = engine.to_device(x, y, z)
+    xo, yo, zo, perm = engine.reorder(dx, dy, dz, p)
+    engine.torch.cuda.synchronize()
+    assert np.array_equal(perm.cpu().numpy().view(np.uint32), perm_ref)
+    for got, src in ((xo, x), (yo, y), (zo, z)):
+        assert np.array_equal(got.cpu().numpy().view(np.uint32), src[perm_ref].view(np.uint32))
+
+
 @pytest.mark.parametrize("name", FORCE_CASES)
 def test_forces_mixed_vs_fp64(engine, name):
     (x, y, z), p = _case(name)
@@ -239,6 +256,61 @@ def test_compat_predicates(engine, pred):
     assert int(res["n_pairs"]) == ref["n_pairs"]
     err = po.rel_errors(f, (ref["fx"], ref["fy"], ref["fz"]))["rel"]
     assert err.max() <= FORCE_TOL
+
+
+def _ray_twins(seed, c):
+    """Uniform particles plus partners placed where the ray models of the Sphere and
+    Triangle programs count a pair 0, 2 or 4 times: within 2e-5 of a particle on one or
+    two of y and z, within c 1e-5 in x, and near-coincident."""
+    x, y, z = ljamd.generate("uniform", seed, 20_000, [0.05, 0.95])
+    rng = np.random.default_rng(seed)
+    base = rng.integers(0, x.size, 3000)
+    off = rng.uniform(-0.9 * c, 0.9 * c, (3000, 3))
+    kind = np.arange(3000) % 4
+    off[kind == 0, 1] = rng.uniform(-2e-5, 2e-5, (kind == 0).sum())
+    off[kind == 1, 1:] = rng.uniform(-2e-5, 2e-5, ((kind == 1).sum(), 2))
+    off[kind == 2, 0] = rng.uniform(-2e-5 * c, 2e-5 * c, (kind == 2).sum())
+    off[kind == 3] = rng.uniform(-3e-5, 3e-5, ((kind == 3).sum(), 3))
+    X = np.concatenate([x, (x[base] + off[:, 0]).astype(np.float32)])
+    Y = np.concatenate([y, (y[base] + off[:, 1]).astype(np.float32)])
+    Z = np.concatenate([z, (z[base] + off[:, 2]).astype(np.float32)])
+    return X, Y, Z
+
+
+@pytest.mark.parametrize("pred", ["sphere", "triangle"])
+def test_ray_models_with_degenerate_pairs(engine, pred):
+    """The Sphere and Triangle programs' ray-hit rules (SDK/optixMultiSphere/optixLJ.cu:
+    77-194, SDK/optixMultiTriangle/optixLJ.cu:82-217) on inputs full of the pairs they
+    count other than once: summed multiplicities (n_pairs) equal the oracle's, the
+    neighbour lists (partners counted at least once) equal the oracle's bit for bit on
+    the cell grid, the refined grid and the LBVH, and forces are within 1e-5 of FP64."""
+    c = 0.06
+    x, y, z = _ray_twins(31, c)
+    p = ljamd.make_params(c, sigma=2e-5, predicate=pred)
+    q = po.from_lj(p)
+    print(f"[ray model] {pred} n={x.size}", flush=True)
+    ref = po.forces_f64(q, x, y, z)
+    rows_o, cols_o = po.pairs_cellgrid(q, x, y, z)
+    if pred == "triangle":
+        assert ref["n_pairs"] > cols_o.size  # some pairs count twice or four times
+    else:  # near-coincident pairs that the AABB test accepts are missed by the rays
+        assert ref["n_pairs"] < po.forces_f64(po.params(c, sigma=2e-5), x, y, z)["n_pairs"]
+    dx, dy, dz = engine.to_device(x, y, z)
+    for lbvh, refine in ((False, 0), (False, 2), (True, 0)):
+        p.refine = refine
+        out = engine.compute(dx, dy, dz, p, lbvh=lbvh)
+        engine.torch.cuda.synchronize()
+        res = ljamd.read_results(out["result"])[0]
+        assert int(res["n_pairs"]) == ref["n_pairs"], (lbvh, refine)
+        f = [out[k][: x.size].cpu().numpy() for k in ("fx", "fy", "fz")]
+        err = po.rel_errors(f, (ref["fx"], ref["fy"], ref["fz"]))["rel"]
+        assert err.max() <= FORCE_TOL, (lbvh, refine, err.max())
+        assert res["e_total"] == pytest.approx(ref["e_total"], rel=1e-9)
+        rows, cols = engine.pairs(dx, dy, dz, p, lbvh=lbvh)
+        engine.torch.cuda.synchronize()
+        assert np.array_equal(rows.cpu().numpy().astype(np.uint64), rows_o)
+        assert np.array_equal(cols.cpu().numpy().view(np.uint32), cols_o)
+    p.refine = 0
 
 
 def test_cudalj_energy_semantics(engine):

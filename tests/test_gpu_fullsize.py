@@ -93,6 +93,44 @@ def test_lbvh_equals_cellgrid_full(engine, name):
     assert np.max(np.abs(le[efin] - ce[efin]) / np.maximum(np.abs(ce[efin]), 1e-30)) <= 1e-6
 
 
+@pytest.mark.parametrize("name", ["c2", "c5"])
+def test_full_pair_lists_cellgrid_equals_lbvh(engine, name):
+    """C2 (1,000,000 points on a sphere) and C5 (4,194,304 clustered particles, about
+    116M ordered pairs): the LBVH's CSR neighbour lists equal the cell grid's bit for
+    bit over every row.  C2's lists equal the oracle's in full, and its pair and
+    candidate counts equal the FP64 oracle's; a 4,096-row subsample of C5 equals the
+    oracle's rows.  The LBVH force call's walks all finish inside their step guard."""
+    w = workloads.single(name)
+    n = w.x.size
+    print(f"[full lists] {name} n={n}", flush=True)
+    q = po.from_lj(w.params)
+    dx, dy, dz = engine.to_device(w.x, w.y, w.z)
+    rows_c, cols_c = engine.pairs(dx, dy, dz, w.params)
+    rows_l, cols_l = engine.pairs(dx, dy, dz, w.params, lbvh=True)
+    engine.torch.cuda.synchronize()
+    rc_, cc_ = rows_c.cpu().numpy(), cols_c.cpu().numpy()
+    assert np.array_equal(rc_, rows_l.cpu().numpy())
+    assert np.array_equal(cc_, cols_l.cpu().numpy())
+    del rows_l, cols_l
+    print(f"[full lists] {name} pairs={cc_.size}", flush=True)
+    (_, _, _, _), lres = _forces(engine, dx, dy, dz, w.params, lbvh=True)
+    assert int(lres["n_pairs"]) == cc_.size and np.isfinite(lres["e_total"])
+    if name == "c2":
+        rows_o, cols_o = po.pairs_cellgrid(q, w.x, w.y, w.z)
+        assert np.array_equal(rc_.astype(np.uint64), rows_o)
+        assert np.array_equal(cc_.view(np.uint32), cols_o)
+        ref = po.forces_f64(q, w.x, w.y, w.z, nthreads=16)
+        (_, _, _, _), cres = _forces(engine, dx, dy, dz, w.params)
+        assert int(cres["n_pairs"]) == ref["n_pairs"] == cc_.size
+        assert int(cres["n_candidates"]) == ref["n_candidates"]
+    else:
+        sub = np.arange(0, n, n // 4096, dtype=np.int64)
+        rows_o, cols_o = po.pairs_rows(q, w.x, w.y, w.z, sub, nthreads=16)
+        got = [cc_[rc_[i]:rc_[i + 1]] for i in sub]
+        assert np.array_equal(np.array([g.size for g in got], np.uint64), np.diff(rows_o))
+        assert np.array_equal(np.concatenate(got).view(np.uint32), cols_o)
+
+
 def test_c4_batched_equals_single_calls_full(engine):
     """All 4,096 systems of the batched workload (46.8M particles): one batched launch
     gives every system the bits and counts of its own call with the same shape.  With

@@ -84,6 +84,31 @@ def test_lbvh_rejects_cudalj(engine):
         engine.compute(dx, dy, dz, p, lbvh=True)
 
 
+def test_lbvh_walk_guard_marks_results(engine):
+    """With every walk capped at one step (LJ_FLAG_WALK_GUARD_TEST, set from the host)
+    the step guard trips: the force call marks its results (n_pairs = UINT64_MAX, NaN
+    totals) and the pair-list call returns LJ_ERR_OVERFLOW.  The next normal call on the
+    same context is unaffected."""
+    (x, y, z), p = _lbvh_case("clusters_256k")
+    dx, dy, dz = engine.to_device(x, y, z)
+    p.flags = ljamd.FLAG_WALK_GUARD_TEST
+    print("[lbvh guard] forced", flush=True)
+    out = engine.compute(dx, dy, dz, p, lbvh=True)
+    engine.torch.cuda.synchronize()
+    res = ljamd.read_results(out["result"])[0]
+    assert int(res["n_pairs"]) == 2**64 - 1
+    assert np.isnan(res["e_total"]) and np.isnan(res["virial"])
+    with pytest.raises(ljamd.LJError) as ei:
+        engine.pairs(dx, dy, dz, p, lbvh=True)
+    assert ei.value.code == -5
+    p.flags = 0
+    out = engine.compute(dx, dy, dz, p, lbvh=True)
+    engine.torch.cuda.synchronize()
+    res = ljamd.read_results(out["result"])[0]
+    ref = po.forces_f64(po.from_lj(p), x, y, z)
+    assert int(res["n_pairs"]) == ref["n_pairs"] and np.isfinite(res["e_total"])
+
+
 def _small_batch(nsys=48):
     return workloads.batch_c4(nsys=nsys, nmin=64, nmax=4096, seed=77)
 
@@ -214,6 +239,64 @@ def test_batched_graph_replays_match_eager(engine):
         torch.cuda.synchronize()
         for k in ("fx", "fy", "fz", "e", "result"):
             assert torch.equal(out[k], eager[k]), (rep, k)
+
+
+def test_workspace_generation_marks_stale_graphs(engine):
+    """A graph captured on a context keeps its workspace addresses.  The generation is
+    unchanged by calls that fit the workspace (replays stay valid and match the eager
+    result) and changes as soon as a larger call reallocates it, which is the caller's
+    signal that the captured graph must not be replayed."""
+    torch = engine.torch
+    systems, X, Y, Z = _small_batch(16)
+    dx, dy, dz = engine.to_device(X, Y, Z)
+    out = engine.compute_batched(systems, dx, dy, dz)
+    torch.cuda.synchronize()
+    eager = out["fx"].clone()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        engine.compute_batched(systems, dx, dy, dz, out=out)
+    torch.cuda.current_stream().wait_stream(s)
+    with torch.cuda.graph(g):
+        engine.compute_batched(systems, dx, dy, dz, out=out)
+    gen = engine.workspace_generation()
+    engine.compute(dx[:300], dy[:300], dz[:300], ljamd.make_params(0.1))  # fits: no reallocation
+    torch.cuda.synchronize()
+    assert engine.workspace_generation() == gen
+    out["fx"].zero_()
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(out["fx"], eager)
+    big = 4 * X.size + 100_000
+    bx, by, bz = engine.to_device(*ljamd.generate("uniform", 99, big))
+    engine.compute(bx, by, bz, ljamd.make_params(0.02))  # larger: the workspace grows
+    torch.cuda.synchronize()
+    assert engine.workspace_generation() > gen
+    del g
+
+
+def test_allocation_failure_reports_oom(monkeypatch):
+    """With device allocations made to fail (LJ_TEST_FAIL_ALLOC_AFTER, read when a
+    context is created) a call that must grow its workspace returns LJ_ERR_OOM and
+    names the allocation; a context created without the variable works again."""
+    x, y, z = ljamd.generate("uniform", 5, 20_000)
+    monkeypatch.setenv("LJ_TEST_FAIL_ALLOC_AFTER", "2")
+    eng = ljamd.Engine(0)
+    try:
+        dx, dy, dz = eng.to_device(x, y, z)
+        with pytest.raises(ljamd.LJError) as ei:
+            eng.compute(dx, dy, dz, ljamd.make_params(0.05))
+        assert ei.value.code == -2 and "grow" in str(ei.value)
+    finally:
+        eng.close()
+        monkeypatch.delenv("LJ_TEST_FAIL_ALLOC_AFTER")
+        eng = ljamd.Engine(0)  # the injection ends with the next context creation
+    out = eng.compute(dx, dy, dz, ljamd.make_params(0.05))
+    eng.torch.cuda.synchronize()
+    res = ljamd.read_results(out["result"])[0]
+    assert int(res["n_pairs"]) == po.forces_f64(po.params(0.05), x, y, z)["n_pairs"]
+    eng.close()
 
 
 def test_batched_rejects_unpacked_systems(engine):

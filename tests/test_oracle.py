@@ -4,6 +4,7 @@ Every check is independent of the oracle's own formulas: numpy's Philox, numpy's
 IEEE float32 division and stable sort, exact rational arithmetic, closed forms of
 the two-body potential, lattice sums, Newton's third law and finite differences.
 """
+import ctypes
 import json
 import math
 import pathlib
@@ -204,6 +205,149 @@ def test_predicate_boundaries():
     assert po.lib().orc_accept(0, 1e-16, 0.25, 0, 1)
     assert not po.lib().orc_accept(4, float(np.float32(1e-4) ** 2), 0.25, 0, 1)  # triangle: d must exceed 1e-4
     assert po.lib().orc_accept(2, 0.0, 0.25, 0, 1) and not po.lib().orc_accept(2, 0.0, 0.25, 3, 3)
+
+
+# ------------------------------------------------------------------ ray-hit models
+def _mult(pred, c, pi, pj):
+    q = po.params(c, predicate=pred)
+    return po.lib().orc_pair_mult(ctypes.byref(q), *(float(v) for v in pi), *(float(v) for v in pj), 0, 1)
+
+
+def _np_sphere_mult(c, pi, pj):
+    """Interval form of the Sphere program's rule, written independently of the
+    oracle's: the pair counts when d2 < c^2 and the ray along the axis closest to p_j
+    reaches the sphere's surface within [0, 2h] (first crossing)."""
+    c = np.float32(c)
+    h = np.float32(c * np.float32(0.81649658092))
+    rad = np.float32(h + np.float32(1e-5))
+    d2 = np.float32(po.d2(pi, pj))
+    if not d2 < np.float32(c * c):
+        return 0
+    sq = [np.float32(abs(np.float32(pi[k] - pj[k]))) ** 2 for k in range(3)]
+    dist_axis = [np.float32(sq[1] + sq[2]), np.float32(sq[0] + sq[2]), np.float32(sq[0] + sq[1])]
+    a = int(np.argmin(dist_axis))  # first minimum: x, then y, then z
+    o_a = np.float32(pi[a] - h)
+    along = float(pj[a]) - float(o_a)
+    perp2 = sum((float(pj[b]) - float(pi[b])) ** 2 for b in range(3) if b != a)
+    disc = float(rad) ** 2 - perp2
+    if disc < 0:
+        return 0
+    t1, t2 = along - math.sqrt(disc), along + math.sqrt(disc)
+    t = t1 if t1 >= 0 else t2
+    return int(0 <= t <= 2.0 * float(h))
+
+
+def _np_triangle_mult(c, pi, pj):
+    """Interval form of the Triangle program's rule: a ray at (y_r, z_r) hits a panel
+    when it lies inside the panel's y and z extents and the panel's x plane is within
+    [tmin, tmax] of the ray origin; each hit counts when the centre q rebuilt from the
+    panel satisfies 1e-4 < |q - p_i| < c and the ray-index rule."""
+    f = np.float32
+    c = f(c)
+    hc = f(c / f(2))
+    hi_off = f(hc + f(1e-5))
+    lo_off = f(-hc - f(1e-5))
+    y_lo, y_hi = f(pj[1] + lo_off), f(pj[1] + hi_off)
+    z_lo, z_hi = f(pj[2] + lo_off), f(pj[2] + hi_off)
+    panels = [(f(pj[0] + (-hc)), 0), (f(pj[0] + hc), 1)]
+    ox = f(pi[0] - hc)
+    tmin = float(f(1.19209290e-07))
+    tmax = float(f(float(f(2) * hc) + float(f(2) * hc) * 0.00001))
+    m = 0
+    for r in range(4):
+        yc, zc = (f(-1) if r & 1 else f(1)), (f(-1) if r & 2 else f(1))
+        yr, zr = f(pi[1] + f(hc * yc)), f(pi[2] + f(hc * zc))
+        if not (y_lo <= yr <= y_hi and z_lo <= zr <= z_hi):
+            continue
+        for xp, back in panels:
+            t = float(xp) - float(ox)
+            if not (tmin <= t <= tmax):
+                continue
+            q = (f(xp - hc) if back else f(xp + hc), f(f(y_lo + y_hi) / f(2)), f(f(z_lo + z_hi) / f(2)))
+            d = f(np.sqrt(np.float32(po.d2(pi, q))))
+            if not (d < c and float(d) > 1e-4):
+                continue
+            ok = ((pi[1] != q[1] and pi[2] != q[2]) or (pi[2] < q[2] and r == 0) or (pi[2] > q[2] and r == 2)
+                  or (pi[1] < q[1] and r == 0) or (pi[1] > q[1] and r == 1) or r == 0)
+            m += int(ok)
+    return m
+
+
+def _ray_pairs(c, n, seed):
+    """Pairs around the rules' edges: generic offsets inside and beyond c, offsets
+    within 2e-5 of zero on one or two axes (several rays or panels), near-coincident
+    pairs, and x offsets within c 1e-5 (both panels)."""
+    rng = np.random.default_rng(seed)
+    pi = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+    off = rng.uniform(-1.3 * c, 1.3 * c, (n, 3))
+    kind = rng.integers(0, 5, n)
+    off[kind == 1, 1] = rng.uniform(-2e-5, 2e-5, (kind == 1).sum())
+    k2 = kind == 2
+    off[k2, 1] = rng.uniform(-2e-5, 2e-5, k2.sum())
+    off[k2, 2] = rng.uniform(-2e-5, 2e-5, k2.sum())
+    k3 = kind == 3
+    off[k3] = rng.uniform(-3e-5, 3e-5, (k3.sum(), 3))
+    off[kind == 4, 0] = rng.uniform(-2e-5 * c, 2e-5 * c, (kind == 4).sum())
+    pj = (pi + off).astype(np.float32)
+    return pi, pj
+
+
+@pytest.mark.parametrize("pred", ["sphere", "triangle"])
+def test_ray_models_match_independent_interval_form(pred):
+    c = 0.05
+    pi, pj = _ray_pairs(c, 3000, 8 if pred == "sphere" else 9)
+    ref = _np_sphere_mult if pred == "sphere" else _np_triangle_mult
+    got = np.array([_mult(pred, c, a, b) for a, b in zip(pi, pj)])
+    want = np.array([ref(c, a, b) for a, b in zip(pi, pj)])
+    assert np.array_equal(got, want), np.flatnonzero(got != want)[:10]
+    if pred == "triangle":
+        assert (got >= 2).sum() > 100 and (got == 4).sum() > 10  # the degenerate families occur
+    else:
+        assert (got == 0).sum() > 100 and got.max() == 1
+
+
+def test_ray_models_hand_placed_pairs():
+    c = 0.05
+    p = np.array([0.4, 0.5, 0.6], np.float32)
+
+    def at(dx, dy, dz):
+        return (p + np.array([dx, dy, dz])).astype(np.float32)
+
+    # Sphere: generic pairs count once inside c; a pair closer than the 1e-5 radius
+    # margin along the ray's axis is missed (the whole ray segment lies inside the sphere)
+    assert _mult("sphere", c, p, at(0.3 * c, 0.2 * c, 0.25 * c)) == 1
+    assert _mult("sphere", c, p, at(0.0, 0.0, 0.999 * c)) == 1
+    assert _mult("sphere", c, p, at(0.0, 0.0, 1.001 * c)) == 0
+    assert _mult("sphere", c, p, at(0.0, 0.0, 1e-6)) == 0
+    assert _mult("sphere", c, p, at(0.0, 0.0, 5e-5)) == 1
+    assert _mult("sphere", c, p, p) == 0
+    # Triangle: one ray and one panel for generic pairs; two rays when |dy| < 1e-5, four
+    # when |dz| < 1e-5 too; both panels when 0 < dx < c 1e-5; none below d = 1e-4
+    assert _mult("triangle", c, p, at(0.3 * c, 0.2 * c, 0.25 * c)) == 1
+    assert _mult("triangle", c, p, at(-0.3 * c, -0.2 * c, -0.25 * c)) == 1
+    assert _mult("triangle", c, p, at(0.3 * c, 5e-6, 0.25 * c)) == 2
+    assert _mult("triangle", c, p, at(0.3 * c, 5e-6, -5e-6)) == 4
+    assert _mult("triangle", c, p, at(2e-7, 0.2 * c, 0.25 * c)) == 2
+    assert _mult("triangle", c, p, at(5e-5, 0.0, 0.0)) == 0
+    assert _mult("triangle", c, p, at(0.3 * c, 0.2 * c, 1.1 * c)) == 0
+
+
+def test_ray_models_reduce_to_distance_predicates_on_generic_pairs():
+    """Away from the degenerate families the Sphere program finds exactly the pairs
+    with d2 < c^2 and the Triangle program those with 1e-4 < d < c, each once."""
+    c = 0.05
+    rng = np.random.default_rng(12)
+    pi = rng.uniform(0.2, 0.8, (4000, 3)).astype(np.float32)
+    off = rng.uniform(-1.2 * c, 1.2 * c, (4000, 3))
+    off = np.where(np.abs(off) < 1e-3, 1e-3, off)  # no axis within 1e-3
+    pj = (pi + off).astype(np.float32)
+    d2 = np.array([po.d2(a, b) for a, b in zip(pi, pj)], np.float32)
+    keep = np.abs(np.sqrt(d2.astype(np.float64)) - c) > 1e-6 * c  # clear of the cutoff
+    for a, b, q, k in zip(pi, pj, d2, keep):
+        if not k:
+            continue
+        assert _mult("sphere", c, a, b) == int(q < np.float32(c * c))
+        assert _mult("triangle", c, a, b) == int(np.sqrt(q) < c)
 
 
 # ------------------------------------------------------------------ FP64 forces
