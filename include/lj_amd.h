@@ -180,6 +180,23 @@ int lj_compute(lj_context* ctx, const float* x, const float* y, const float* z, 
                const lj_params* p, float* fx, float* fy, float* fz, float* e,
                lj_result* result, void* stream);
 
+/* Device: lj_compute without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate during a call (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels and results
+ * as lj_compute, bit for bit; the workspace is a fixed array of
+ * LJ_NOALLOC_SCRATCH_BYTES in device memory, one per GPU, held until the process ends
+ * and shared by every call on that GPU, so calls that overlap on one GPU (other
+ * streams, threads or contexts) race and must be serialised by the caller.  Inputs of
+ * more than LJ_NOALLOC_MAX_PARTICLES particles or a force grid (the kernel shape's
+ * refined grid) of more than LJ_NOALLOC_MAX_CELLS cells return LJ_ERR_CAPACITY.
+ * Profiling (lj_profile_enable) does not record this call. */
+#define LJ_NOALLOC_MAX_PARTICLES (1 << 20)
+#define LJ_NOALLOC_MAX_CELLS (1 << 21)
+#define LJ_NOALLOC_SCRATCH_BYTES (64u << 20)
+int lj_compute_noalloc(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n,
+                       const lj_params* p, float* fx, float* fy, float* fz, float* e,
+                       lj_result* result, void* stream);
+
 /* Device: neighbour lists from the cell grid, CSR by original index.
  *   row_offsets[n+1] (device u64), cols[capacity] (device u32, original indices,
  *   each row sorted ascending).  *needed (host) receives the pair count; when it

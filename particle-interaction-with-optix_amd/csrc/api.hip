@@ -12,6 +12,7 @@ namespace lj {
 static thread_local std::string g_create_err;
 
 std::atomic<long long> g_alloc_budget{-1};
+thread_local Arena* tl_arena = nullptr;
 
 // Every device buffer a context owns (the workspace).
 template <class F>
@@ -198,6 +199,12 @@ int lj_context_create(lj_context** out, int device) {
     if (!ctx) return fail(nullptr, LJ_ERR_OOM, "host allocation failed");
     ctx->device = device;
     ctx->num_cus = prop.multiProcessorCount;
+    const int sr = scratch_setup(ctx, device);
+    if (sr != LJ_OK) {
+        fail(nullptr, sr, "%s", ctx->err.c_str());
+        delete ctx;
+        return sr;
+    }
     // test hook: LJ_TEST_FAIL_ALLOC_AFTER=k lets k more device allocations succeed
     const char* inj = std::getenv("LJ_TEST_FAIL_ALLOC_AFTER");
     g_alloc_budget.store(inj && *inj ? std::strtoll(inj, nullptr, 10) : -1, std::memory_order_relaxed);

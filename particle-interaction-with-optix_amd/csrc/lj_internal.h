@@ -29,6 +29,14 @@ inline bool alloc_fault_injected() {
     return b == 0;
 }
 
+// A fixed block of device memory that grow() carves buffers from, front to back,
+// instead of allocating (lj_compute_noalloc points tl_arena at it for one call).
+struct Arena {
+    unsigned char* base;
+    size_t size, used;
+};
+extern thread_local Arena* tl_arena;
+
 // A grow-only device buffer.  grow() reallocates only when the request exceeds the
 // current capacity, so steady-state calls of one size allocate nothing.  `reallocs`
 // counts the allocations, so a caller holding the buffer's address (a captured
@@ -39,6 +47,15 @@ struct DevBuf {
     uint64_t reallocs = 0;
     hipError_t grow(size_t need) {
         if (need <= bytes) return hipSuccess;
+        if (Arena* a = tl_arena) {
+            // carve from the arena; the carved buffers are dropped with the call
+            const size_t cap = ((need < 256 ? 256 : need) + 255) / 256 * 256;
+            if (cap > a->size - a->used) return hipErrorOutOfMemory;
+            ptr = a->base + a->used;
+            a->used += cap;
+            bytes = cap;
+            return hipSuccess;
+        }
         if (alloc_fault_injected()) return hipErrorOutOfMemory;
         if (ptr) {
             hipError_t e = hipFree(ptr);
@@ -131,11 +148,16 @@ struct lj_context {
     lj::LbvhWork lbvh;
     lj::BatchWork batch;
     lj::DevBuf tune_out;  // outputs of the autotuner's timed calls
+    void* scratch = nullptr;  // device address of this device's lj_compute_noalloc array
 };
 
 namespace lj {
 
 int fail(lj_context* ctx, int code, const char* fmt, ...);
+
+// Looks up (once per device) the device address of lj_compute_noalloc's scratch array
+// and stores it in ctx->scratch (noalloc.hip).
+int scratch_setup(lj_context* ctx, int device);
 
 #define LJ_HIP(ctx, expr)                                                                   \
     do {                                                                                    \

@@ -36,6 +36,8 @@ PRED = {"aabb": 0, "custom": 1, "cudalj": 2, "sphere": 3, "triangle": 4}
 PREC = {"mixed": 0, "fp32": 1, "fp32_comp": 2}
 FLAG_NO_VIRIAL = 1
 FLAG_WALK_GUARD_TEST = 2
+NOALLOC_MAX_PARTICLES = 1 << 20  # lj_amd.h LJ_NOALLOC_MAX_PARTICLES
+NOALLOC_MAX_CELLS = 1 << 21
 
 GEN = {"uniform": 0, "surface": 1, "clusters": 2}
 
@@ -61,6 +63,7 @@ EXPORTS = (
     "lj_compute_list",
     "lj_max_displacement",
     "lj_workspace_generation",
+    "lj_compute_noalloc",
 )
 PHASES = ("bin", "force", "other")
 
@@ -166,6 +169,7 @@ def load(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
         "lj_compute_list": ([vp, vp, vp, vp, i64, P(LJParams), vp, vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "lj_max_displacement": ([vp, vp, vp, vp, vp, vp, vp, i64, vp, vp], ctypes.c_int),
         "lj_workspace_generation": ([vp, P(u64)], ctypes.c_int),
+        "lj_compute_noalloc": ([vp, vp, vp, vp, i64, P(LJParams), vp, vp, vp, vp, vp, vp], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         if hasattr(lib, name):  # tests check that every declared symbol is exported
@@ -356,6 +360,16 @@ class Engine:
         self._check(
             fn(self.ctx, _ptr(x), _ptr(y), _ptr(z), n, ctypes.byref(p), _ptr(out["fx"]), _ptr(out["fy"]), _ptr(out["fz"]),
                _ptr(out["e"]), _ptr(out["result"]), self._stream())
+        )
+        return out
+
+    def compute_noalloc(self, x, y, z, p: LJParams, out):
+        """lj_compute_noalloc: compute() with the workspace in the library's fixed device
+        array (no allocation); `out` holds fx, fy, fz, e (nullable) and result."""
+        self._check(
+            self.lib.lj_compute_noalloc(self.ctx, _ptr(x), _ptr(y), _ptr(z), x.numel(), ctypes.byref(p), _ptr(out["fx"]),
+                                        _ptr(out["fy"]), _ptr(out["fz"]), _ptr(out["e"]), _ptr(out["result"]),
+                                        self._stream())
         )
         return out
 
