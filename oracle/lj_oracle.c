@@ -257,7 +257,10 @@ static int sphere_mult(float c, const float pi[3], const float pj[3]) {
 }
 
 /* Moller & Trumbore, "Fast, minimum storage ray-triangle intersection" (1997), in
- * FP64, for a ray with direction (1, 0, 0); edges and vertices count as inside. */
+ * FP64, for a ray with direction (1, 0, 0); edges and vertices count as inside.  For a
+ * triangle in a plane x = const (the panels) the hit distance is taken exactly, as the
+ * plane's x minus the origin's (both FP32, so the FP64 difference is exact), rather
+ * than from the rounded determinant ratio: the t range test then has one answer. */
 static int ray_triangle(const float o[3], const float v0[3], const float v1[3], const float v2[3], double tmin,
                         double tmax) {
     const double e1[3] = {(double)v1[0] - v0[0], (double)v1[1] - v0[1], (double)v1[2] - v0[2]};
@@ -272,7 +275,8 @@ static int ray_triangle(const float o[3], const float v0[3], const float v1[3], 
     const double qv[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
     const double v = qv[0] / det; /* dir . qvec */
     if (v < 0.0 || u + v > 1.0) return 0;
-    const double t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) / det;
+    const double t = (v0[0] == v1[0] && v1[0] == v2[0]) ? (double)v0[0] - (double)o[0]
+                                                          : (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) / det;
     return t >= tmin && t <= tmax;
 }
 

@@ -144,7 +144,7 @@ constexpr uint32_t kSharedWave = 0xffffffffu;
 template <int PRED, int PREC, bool QUEUE>
 __device__ inline void batch_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
                                    bool active, uint32_t key, const SysDesc& d, int Rx, int R, uint32_t depth,
-                                   uint32_t* __restrict__ queue, uint32_t stride, float* __restrict__ fx,
+                                   QEntry* __restrict__ queue, float* __restrict__ fx,
                                    float* __restrict__ fy, float* __restrict__ fz, float* __restrict__ e, double& ei,
                                    double& vi, uint32_t& np) {
     const GridGeom g = d.g;
@@ -155,7 +155,7 @@ __device__ inline void batch_pairs(const float4* __restrict__ pos, const uint32_
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
     acc.allow_exact_diff(c, pi, active);
-    np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, Rx, R, tr, c, queue, stride,
+    np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, Rx, R, tr, c, queue,
                                                 depth, acc);
     ei = vi = 0.0;
     if (active) {
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
                                                            int R, uint32_t depth, float* __restrict__ fx,
                                                            float* __restrict__ fy, float* __restrict__ fz,
                                                            float* __restrict__ e, WaveTotals* __restrict__ wrec) {
-    extern __shared__ uint32_t queue[];
+    extern __shared__ QEntry queue[];
     const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = k < n;
     const uint32_t key = active ? cell_of[k] : 0u;
@@ -196,7 +196,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
     if (s0 == kSharedWave || !__all(!active || si == s0)) return;
     double ei, vi;
     uint32_t np;
-    batch_pairs<PRED, PREC, QUEUE>(pos, off, k, active, key, sys[s0], Rx, R, depth, queue + threadIdx.x, blockDim.x, fx,
+    batch_pairs<PRED, PREC, QUEUE>(pos, off, k, active, key, sys[s0], Rx, R, depth,
+                                   queue + (threadIdx.x / kWaveSize) * depth * kWaveSize + __lane_id(), fx,
                                    fy, fz, e, ei, vi, np);
     const double es = wave_sum(ei), vs = wave_sum(vi);
     const unsigned long long ns = wave_sum((unsigned long long)np);
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(kWaveSize) void k_batch_force_shared(
     uint32_t nsys, const uint32_t* __restrict__ shared_waves, int Rx, int R, uint32_t depth, float* __restrict__ fx,
     float* __restrict__ fy, float* __restrict__ fz, float* __restrict__ e, double2* __restrict__ ev,
     uint32_t* __restrict__ npair, WaveTotals* __restrict__ wrec) {
-    extern __shared__ uint32_t queue[];
+    extern __shared__ QEntry queue[];
     const uint32_t w = shared_waves[blockIdx.x];
     const uint32_t k = w * kWaveSize + threadIdx.x;
     const bool active = k < n;
@@ -221,7 +222,8 @@ __global__ __launch_bounds__(kWaveSize) void k_batch_force_shared(
     double ei, vi;
     uint32_t np;
     batch_pairs<PRED, PREC, QUEUE>(pos, off, k, active, key, sys[find_sys(cbase, chint, nsys, key)], Rx, R, depth,
-                                   queue + threadIdx.x, blockDim.x, fx, fy, fz, e, ei, vi, np);
+                                   queue + (threadIdx.x / kWaveSize) * depth * kWaveSize + __lane_id(), fx, fy, fz, e,
+                                   ei, vi, np);
     if (active) {
         ev[k] = make_double2(ei, vi);
         npair[k] = np;
@@ -429,7 +431,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
         const unsigned nb = div_up(n, sh.block);
         const size_t lds = sh.lds_bytes();
-        const size_t lds_shared = sh.queued() ? (size_t)sh.depth * kWaveSize * sizeof(uint32_t) : 0;
+        const size_t lds_shared = sh.queued() ? (size_t)sh.depth * kWaveSize * sizeof(QEntry) : 0;
         if (n > 0) {
 #define LJ_BFQ(PRED, PREC, Q)                                                                                     \
     do {                                                                                                          \

@@ -20,7 +20,8 @@ namespace lj {
 namespace {
 
 // One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues in
-// dynamic LDS (depth x block words; none without QUEUE).  Outputs go to the caller's
+// dynamic LDS (depth x block entries of 12 bytes, one block of rings per wave; none
+// without QUEUE).  Outputs go to the caller's
 // order (pi.w holds the original index); per-block FP64 energy and virial partials are
 // reduced later in a fixed order.
 template <int PRED, int PREC, bool VIR, bool QUEUE>
@@ -30,7 +31,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ 
                                                      float* __restrict__ fy, float* __restrict__ fz,
                                                      float* __restrict__ e, double* __restrict__ partials,
                                                      unsigned long long* __restrict__ npairs) {
-    extern __shared__ uint32_t queue[];
+    extern __shared__ QEntry queue[];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t k = blk * blockDim.x + threadIdx.x;
     const bool active = k < n;
@@ -39,7 +40,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ 
     acc.set_i(pi);
     acc.allow_exact_diff(c, pi, active);
     const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u,
-                                                              Rx, R, tr, c, queue + threadIdx.x, blockDim.x, depth, acc);
+                                                              Rx, R, tr, c,
+                                                              queue + (threadIdx.x / kWaveSize) * depth * kWaveSize +
+                                                                  __lane_id(),
+                                                              depth, acc);
     double esum = 0.0, vsum = 0.0;
     if (active) {
         const uint32_t i = __float_as_uint(pi.w);

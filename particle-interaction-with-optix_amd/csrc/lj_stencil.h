@@ -37,7 +37,7 @@ struct Trim {
 struct Shape {
     int refine, refine_x, block, depth;
     bool queued() const { return depth > 1; }
-    size_t lds_bytes() const { return queued() ? (size_t)depth * (size_t)block * sizeof(uint32_t) : 0; }
+    size_t lds_bytes() const { return queued() ? (size_t)depth * (size_t)block * 16 : 0; }
 };
 
 constexpr int kMaxBlock = 256;
@@ -92,46 +92,48 @@ __device__ inline uint32_t coarse_count(const uint32_t* __restrict__ foff, const
 // Pairs.  With QUEUE = false (the default shape) the FP64 pair arithmetic of an
 // accepted candidate runs inside the candidate loop; it then runs for the whole wave
 // whenever any lane accepts, with about a quarter of the lanes busy.  With QUEUE =
-// true the candidate loop only evaluates the predicate: an accepted partner's slot
-// goes to the lane's queue in LDS (entry q of the lane at queue[q * stride]: lanes of
-// a wave store to consecutive words, never a bank conflict), and the pair arithmetic
-// runs when some lane's queue is nearly full, on every lane's queue at once, in queue
-// order (same order, same bits).  On MI355X the queue was slower on every workload
-// measured -- lanes fill their queues at different rates, a flush runs as long as the
-// fullest queue, and it reloads the queued partners -- so it is a tuning option, not
-// the default.  Every lane of the wave must call this (inactive lanes with active =
-// false), because the flushes are wave-wide.
+// true the candidate loop only evaluates the predicate and appends an accepted
+// partner's position to the lane's ring of `depth` entries in LDS.  The rings of a
+// wave are one block of depth x 64 x 16 bytes (entry q of lane l at (q 64 + l) 16), so
+// an entry is one 128-bit access at a shifted address and the lanes of a wave use
+// consecutive words.  A
+// drain step pops the oldest entry of every lane that has one and runs the pair
+// arithmetic on it; each candidate iteration drains once when at least kDrainLanes
+// lanes hold entries, and up to kCandidates times when a ring could not take the
+// iteration's candidates.  Entries are popped in the order they were pushed, so the
+// sums and their bits are those of QUEUE = false.  Every lane of the wave must call
+// this (inactive lanes with active = false), because the drains are wave-wide.
+constexpr int kDrainLanes = 40;
+
+struct __align__(16) QEntry {
+    float x, y, z, pad;
+};
+
 template <int PRED, int PREC, bool VIR, bool QUEUE>
 __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
                                          bool active, const float4& pi, const GridGeom& g, uint32_t cbase,
                                          uint32_t key, int Rx, int R, const Trim& tr, const PairConst& c,
-                                         uint32_t* __restrict__ queue, uint32_t stride, unsigned depth,
+                                         QEntry* __restrict__ queue, unsigned depth,
                                          PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR>& acc) {
     // only the TRIANGLE ray model counts a pair more than once (queued partners then
-    // get their multiplicity again when flushed)
+    // get their multiplicity again when drained)
     constexpr bool kMultiple = PRED == LJ_PRED_TRIANGLE;
-    unsigned pc = 0;          // queued partners of this lane
-    uint32_t* tail = queue;   // = queue + pc * stride
+    unsigned pc = 0, head = 0;  // entries queued by this lane, index of the oldest
     uint32_t np = 0;
-    const uint32_t kload = active ? k : 0;  // an in-bounds slot for idle loads
-    auto flush = [&]() {
-        for (unsigned q = 0; __any(q < pc); q += 2) {
-            const bool a0 = q < pc, a1 = q + 1 < pc;
-            const float4 p0 = pos[a0 ? queue[q * stride] : kload];
-            const float4 p1 = pos[a1 ? queue[(q + 1) * stride] : kload];
-            if (a0) {
-                const float dx = p0.x - pi.x, dy = p0.y - pi.y, dz = p0.z - pi.z;
-                const float d2 = dist2(dx, dy, dz);
-                acc.add(c, p0, dx, dy, dz, d2, kMultiple ? pair_mult<PRED>(c, pi, p0, d2) : 1);
-            }
-            if (a1) {
-                const float dx = p1.x - pi.x, dy = p1.y - pi.y, dz = p1.z - pi.z;
-                const float d2 = dist2(dx, dy, dz);
-                acc.add(c, p1, dx, dy, dz, d2, kMultiple ? pair_mult<PRED>(c, pi, p1, d2) : 1);
-            }
+    auto push = [&](const float4& pj) {
+        queue[((head + pc) & (depth - 1)) * kWaveSize] = QEntry{pj.x, pj.y, pj.z, 0.f};
+        ++pc;
+    };
+    auto drain_one = [&]() {
+        if (pc > 0) {
+            const QEntry q = queue[head * kWaveSize];
+            const float4 pj = make_float4(q.x, q.y, q.z, 0.f);
+            const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const float d2 = dist2(dx, dy, dz);
+            acc.add(c, pj, dx, dy, dz, d2, kMultiple ? pair_mult<PRED>(c, pi, pj, d2) : 1);
+            head = (head + 1) & (depth - 1);
+            --pc;
         }
-        pc = 0;
-        tail = queue;
     };
     auto visit = [&](uint32_t t, const float4& pj) {
         const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
@@ -145,9 +147,7 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
         if (!mult) return;
         np += (uint32_t)mult;
         if constexpr (QUEUE) {
-            *tail = t;
-            tail += stride;
-            ++pc;
+            push(pj);
         } else {
             acc.add(c, pj, dx, dy, dz, d2, mult);
         }
@@ -157,7 +157,7 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
     const bool trim = active && tr.on && pi.x >= g.lo[0] && pi.y >= g.lo[1] && pi.z >= g.lo[2] &&
                       pi.x <= tr.hi[0] && pi.y <= tr.hi[1] && pi.z <= tr.hi[2];
     const int32_t x0 = cx - Rx > 0 ? cx - Rx : 0, x1 = cx + Rx < g.g[0] - 1 ? cx + Rx : g.g[0] - 1;
-    // an inactive lane has an empty stencil (z0 > z1) but still takes part in the flushes
+    // an inactive lane has an empty stencil (z0 > z1) but still takes part in the drains
     const int32_t z0 = active ? (cz - R > 0 ? cz - R : 0) : 1;
     const int32_t z1 = active ? (cz + R < g.g[2] - 1 ? cz + R : g.g[2] - 1) : 0;
     const int32_t y0 = cy - R > 0 ? cy - R : 0, y1 = cy + R < g.g[1] - 1 ? cy + R : g.g[1] - 1;
@@ -193,6 +193,9 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
     // times than a single row, and a wave runs each step for its longest lane.
     for (int32_t dz = 0; dz <= R; ++dz)
         for (int32_t dy = dz == 0 ? 0 : -R; dy <= R; ++dy) {
+            if constexpr (QUEUE) {  // every lane takes part here
+                while (__popcll(__ballot(pc > 0)) >= (unsigned)kDrainLanes) drain_one();
+            }
             uint32_t t, en, b2, e2;
             row_range(cz + dz, cy + dy, t, en);
             if (dz == 0 && dy == 0) b2 = e2 = 0u;
@@ -207,8 +210,15 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
 #pragma unroll
             for (int u = 1; u < kCandidates; ++u) enu[u] = en > (uint32_t)u ? en - (uint32_t)u : 0u;
             while (t < en) {
-                if constexpr (QUEUE)
-                    if (__any(pc + kCandidates > depth)) flush();
+                if constexpr (QUEUE) {
+                    // room for this iteration's candidates in every walking lane's ring
+                    // (a lane pushes at most kCandidates per iteration), then one step
+                    // when enough lanes are waiting
+#pragma unroll
+                    for (int u = 0; u < kCandidates; ++u)
+                        if (__any(pc + kCandidates > depth)) drain_one();
+                    if (__popcll(__ballot(pc > 0)) >= (unsigned)kDrainLanes) drain_one();
+                }
                 // slots up to t + kCandidates - 1 <= n + kCandidates - 2 exist (pos has
                 // spare entries); they count only below en.  One 64-bit base, the
                 // candidates at immediate offsets from it.
@@ -230,9 +240,9 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
                 }
             }
         }
-    // rows end at different times in different lanes: flush while any queue holds pairs
+    // rows end at different times in different lanes: drain while any ring holds pairs
     if constexpr (QUEUE)
-        while (__any(pc > 0)) flush();
+        while (__any(pc > 0)) drain_one();
     return np;
 }
 

@@ -12,6 +12,12 @@
 //       force; with -rb the uniform (boxDiv <= 32) or surface (boxDiv <= 256) sweep,
 //       5 core() calls per point, CSV results-spherecustom[sorted][gensurface]-*.csv.
 //       The BVH build and traversal are the engine's LBVH (lj_lbvh_compute).
+//   ljbench custom|sphere|triangle [-rb] [-gs] [-np|-ns N] [-bd D] [--max-boxdiv M]
+//       the Custom, Sphere and Triangle drivers (SDK/optixMultiCustom/optixMultiCustom.cpp:
+//       674-861, SDK/optixMultiSphere/optixSphere.cpp:843-1025, SDK/optixMultiTriangle/
+//       optixTriangle.cpp:881-1062): the same runs with each program's pair rule (the
+//       LJ_PRED_CUSTOM distance test, the SPHERE and TRIANGLE ray-hit models), CSV
+//       results-{custom,sphere,triangle}[-surface]-*.csv.
 //   ljbench lj --n N --rc RC [--gen uniform|surface|clusters] [--loops L] [--refine K]
 //              [--precision mixed|fp32|fp32_comp] [--lbvh] [--seed S]
 //       One system with the engine's defaults (sigma = rc/2.5, eps = 1, AABB cutoff,
@@ -229,13 +235,31 @@ int run_cudalj(bool gensurface, int max_boxdiv) {
     return 0;
 }
 
-// ------------------------------------------------------------------ AABB mode
-// core(nbSpheres, sphereRadius, ..., checkResult, sortParticles, gensurface)
-// (SDK/optixMultiSphereAABB/optixSphere.cpp:82-584): generate, optionally sort into
-// cell order, build the hierarchy, evaluate every particle's energy with the AABB
-// predicate (sigma = eps = 1), optionally check the first 100 against a brute force.
-std::pair<double, double> core(lj_context* ctx, Device& dev, int n, float radius, bool check, bool sort, bool gensurface,
-                               uint64_t stream) {
+// ------------------------------------------------------------------ OptiX program modes
+// The four OptiX drivers share one structure and differ in their pair rule, their CSV
+// name and the spelling of the particle-count flag.
+struct Program {
+    const char* name;  // CLI mode
+    int predicate;     // the program's pair rule (lj_amd.h)
+    bool aabb;         // the AABB driver: -us pre-sort, its own CSV name and checked-run test
+};
+
+std::string csv_name(const Program& prog, bool sort, bool gensurface) {
+    if (prog.aabb)  // SDK/optixMultiSphereAABB/optixSphere.cpp getFilename
+        return std::string("results-spherecustom") + (sort ? "sorted" : "") + (gensurface ? "gensurface" : "") + "-" +
+               stamp() + ".csv";
+    // SDK/optixMultiCustom/optixMultiCustom.cpp:630-644 and the Sphere / Triangle copies
+    return std::string("results-") + prog.name + (gensurface ? "-surface" : "") + "-" + stamp() + ".csv";
+}
+
+// core(nbPoints, cutoffRadius, ..., checkResult, [sortParticles,] gensurface)
+// (SDK/optixMultiSphereAABB/optixSphere.cpp:82-584, optixMultiCustom.cpp:108-620,
+// optixMultiSphere/optixSphere.cpp:94-785, optixMultiTriangle/optixTriangle.cpp:110-825):
+// generate, optionally sort into cell order, build the hierarchy, evaluate every
+// particle's energy with the program's pair rule (sigma = eps = 1), optionally check
+// the first 100 against the driver's host brute force.
+std::pair<double, double> core(lj_context* ctx, Device& dev, const Program& prog, int n, float radius, bool check,
+                               bool sort, bool gensurface, uint64_t stream) {
     std::vector<float> x((size_t)n), y((size_t)n), z((size_t)n);
     float lo = 0.0f, hi = 1.0f;
     if (gensurface) {
@@ -246,7 +270,7 @@ std::pair<double, double> core(lj_context* ctx, Device& dev, int n, float radius
     } else {
         CHECK_LJ(ctx, lj_generate(LJ_GEN_UNIFORM, 48, stream, n, nullptr, x.data(), y.data(), z.data()));
     }
-    lj_params p = base_params(radius, 1.0f, 1.0f, lo, hi, LJ_PRED_AABB);
+    lj_params p = base_params(radius, 1.0f, 1.0f, lo, hi, prog.predicate);
     dev.reserve((size_t)n);
     upload(dev, x, y, z);
     double tsort = 0.0;
@@ -276,14 +300,16 @@ std::pair<double, double> core(lj_context* ctx, Device& dev, int n, float radius
     if (check) {
         std::vector<float> e((size_t)n);
         CHECK_HIP(hipMemcpy(e.data(), dev.e, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
-        // checkResult of optixSphere.cpp:550-575: host brute force over every other particle
+        // checkResult (optixSphere.cpp:550-575; the other drivers test dist < c, e.g.
+        // optixMultiCustom.cpp:593-616): host brute force over every other particle
         for (int i = 0; i < std::min(100, n); ++i) {
             double ref = 0.0;
             for (int j = 0; j < n; ++j) {
                 if (j == i) continue;
                 const double dx = (double)x[j] - x[i], dy = (double)y[j] - y[i], dz = (double)z[j] - z[i];
                 const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
-                if (d <= radius && d * d >= 1e-16) ref += 4.0 * (std::pow(1.0 / d, 12) - std::pow(1.0 / d, 6));
+                const bool in = prog.aabb ? (d <= radius && d * d >= 1e-16) : d < radius;
+                if (in) ref += 4.0 * (std::pow(1.0 / d, 12) - std::pow(1.0 / d, 6));
             }
             std::printf("Energy for particle %d position %.9g %.9g %.9g is %.17g it has been computed as %.9g\n", i,
                         (double)x[i], (double)y[i], (double)z[i], ref, (double)e[(size_t)i]);
@@ -292,16 +318,17 @@ std::pair<double, double> core(lj_context* ctx, Device& dev, int n, float radius
     return {tsort + t.init, t.compute};
 }
 
-int run_aabb(bool rb, bool gensurface, bool sort, int ns, int bd, int max_div) {
+int run_optix(const Program& prog, bool rb, bool gensurface, bool sort, int ns, int bd, int max_div) {
     lj_context* ctx = nullptr;
     CHECK_LJ(nullptr, lj_context_create(&ctx, 0));
     Device dev;
+    if (!prog.aabb) sort = false;  // only the AABB driver has the pre-sort
     if (!rb) {
         const float radius = (float)(1.0 / bd);
         std::printf("[LOG] nb spheres = %d\n[LOG] sphere radius = %g\n[LOG] box divisor = %d\n", ns, (double)radius, bd);
-        core(ctx, dev, ns, radius, true, true, gensurface, 0);
+        core(ctx, dev, prog, ns, radius, true, prog.aabb, gensurface, 0);
     } else {
-        core(ctx, dev, 10, 0.5f, false, sort, gensurface, 0);  // warm-up run, as the reference
+        core(ctx, dev, prog, 10, 0.5f, false, sort, gensurface, 0);  // warm-up run, as the reference
         std::vector<Frame> all;
         const int loops = 5, max_ppc = 32;
         const int top = std::min(max_div, gensurface ? 256 : 32);
@@ -321,14 +348,15 @@ int run_aabb(bool rb, bool gensurface, bool sort, int ns, int bd, int max_div) {
                 // the reference's formula N * (N / nbBoxes) * 27 (optixSphere.cpp:741), in 64 bits
                 fr.nb_interactions = (long long)n * (n / nb_boxes) * 27;
                 for (int l = 0; l < loops; ++l) {
-                    auto [ti, tc] = core(ctx, dev, n, radius, false, sort, gensurface, (uint64_t)(l + 1));
+                    // (the Custom and Triangle surface sweeps of the reference pass the -np
+                    // count instead of N here, optixMultiCustom.cpp:769; every point runs N)
+                    auto [ti, tc] = core(ctx, dev, prog, n, radius, false, sort, gensurface, (uint64_t)(l + 1));
                     fr.results.push_back({ti, tc});
                 }
                 all.push_back(fr);
             }
         }
-        const std::string fname = std::string("results-spherecustom") + (sort ? "sorted" : "") +
-                                  (gensurface ? "gensurface" : "") + "-" + stamp() + ".csv";
+        const std::string fname = csv_name(prog, sort, gensurface);
         std::ofstream f(fname);
         f << "NbParticles,NbInteractions,NbLoops,boxDiv,nbCells,partspercell,timeinit,timecompute,timetotal\n";
         for (const Frame& fr : all) {
@@ -429,11 +457,13 @@ void usage(const char* argv0) {
                  "Usage: %s cudalj [-gs|--gensurface] [--max-boxdiv M]\n"
                  "       %s aabb [-rb|--runbench] [-gs|--gensurface] [-us|--usedsort] [-ns|--nbspheres N]\n"
                  "            [-bd|--boxdivisor D] [--max-boxdiv M]\n"
+                 "       %s custom|triangle [-rb] [-gs] [-np|--nbpoints N] [-bd D] [--max-boxdiv M]\n"
+                 "       %s sphere [-rb] [-gs] [-ns|--nbspheres N] [-bd D] [--max-boxdiv M]\n"
                  "       %s lj [--n N] [--rc RC] [--gen uniform|surface|clusters] [--loops L] [--refine K]\n"
                  "            [--refine-x KX]\n"
                  "            [--block B] [--depth D] [--autotune] [--precision mixed|fp32|fp32_comp] [--lbvh]\n"
                  "            [--seed S]\n",
-                 argv0, argv0, argv0);
+                 argv0, argv0, argv0, argv0, argv0);
     std::exit(1);
 }
 
@@ -450,7 +480,8 @@ int main(int argc, char** argv) {
         if (a == "-gs" || a == "--gensurface") gensurface = true;
         else if (a == "-rb" || a == "--runbench") rb = true;
         else if (a == "-us" || a == "--usedsort") sort = true;
-        else if ((a == "-ns" || a == "--nbspheres") && i + 1 < argc) ns = std::atoi(argv[++i]);
+        else if ((a == "-ns" || a == "--nbspheres" || a == "-np" || a == "--nbpoints") && i + 1 < argc)
+            ns = std::atoi(argv[++i]);
         else if ((a == "-bd" || a == "--boxdivisor") && i + 1 < argc) bd = std::atoi(argv[++i]);
         else if (a == "--max-boxdiv" && i + 1 < argc) max_div = std::atoi(argv[++i]);
         else if (a == "-h" || a == "--help") usage(argv[0]);
@@ -460,6 +491,11 @@ int main(int argc, char** argv) {
         }
     }
     if (mode == "cudalj") return run_cudalj(gensurface, max_div);
-    if (mode == "aabb") return run_aabb(rb, gensurface, sort, ns, bd, max_div);
+    static const Program programs[] = {{"aabb", LJ_PRED_AABB, true},
+                                        {"custom", LJ_PRED_CUSTOM, false},
+                                        {"sphere", LJ_PRED_SPHERE, false},
+                                        {"triangle", LJ_PRED_TRIANGLE, false}};
+    for (const Program& prog : programs)
+        if (mode == prog.name) return run_optix(prog, rb, gensurface, sort, ns, bd, max_div);
     usage(argv[0]);
 }

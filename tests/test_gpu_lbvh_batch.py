@@ -241,15 +241,18 @@ def test_batched_graph_replays_match_eager(engine):
             assert torch.equal(out[k], eager[k]), (rep, k)
 
 
-def test_workspace_generation_marks_stale_graphs(engine):
+def test_workspace_generation_marks_stale_graphs():
     """A graph captured on a context keeps its workspace addresses.  The generation is
     unchanged by calls that fit the workspace (replays stay valid and match the eager
     result) and changes as soon as a larger call reallocates it, which is the caller's
-    signal that the captured graph must not be replayed."""
+    signal that the captured graph must not be replayed.  (A context of its own, so that
+    earlier tests have not grown its workspace already.)"""
+    engine = ljamd.Engine(0)
     torch = engine.torch
     systems, X, Y, Z = _small_batch(16)
     dx, dy, dz = engine.to_device(X, Y, Z)
     out = engine.compute_batched(systems, dx, dy, dz)
+    engine.compute(dx[:300], dy[:300], dz[:300], ljamd.make_params(0.1))  # grows what this call needs
     torch.cuda.synchronize()
     eager = out["fx"].clone()
     g = torch.cuda.CUDAGraph()
@@ -274,6 +277,7 @@ def test_workspace_generation_marks_stale_graphs(engine):
     torch.cuda.synchronize()
     assert engine.workspace_generation() > gen
     del g
+    engine.close()
 
 
 def test_allocation_failure_reports_oom(monkeypatch):
