@@ -160,11 +160,11 @@ __device__ inline void batch_pairs(const float4* __restrict__ pos, const uint32_
     ei = vi = 0.0;
     if (active) {
         const uint32_t i = __float_as_uint(pi.w);
-        fx[i] = (float)(-c.e24 * (double)acc.ax.value());
-        fy[i] = (float)(-c.e24 * (double)acc.ay.value());
-        fz[i] = (float)(-c.e24 * (double)acc.az.value());
+        stream_store(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
+        stream_store(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
+        stream_store(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
         ei = c.e4 * (double)acc.ae.value();
-        if (e) e[i] = (float)ei;
+        if (e) stream_store(&e[i], (float)ei);
         vi = c.e24 * (double)acc.av.value();
     }
 }

@@ -25,7 +25,7 @@ namespace {
 // order (pi.w holds the original index); per-block FP64 energy and virial partials are
 // reduced later in a fixed order.
 template <int PRED, int PREC, bool VIR, bool QUEUE>
-__global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
+__global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
                                                      const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int Rx,
                                                      int R, Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
                                                      float* __restrict__ fy, float* __restrict__ fz,
@@ -47,11 +47,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_force(const float4* __restrict__ 
     double esum = 0.0, vsum = 0.0;
     if (active) {
         const uint32_t i = __float_as_uint(pi.w);
-        fx[i] = (float)(-c.e24 * (double)acc.ax.value());
-        fy[i] = (float)(-c.e24 * (double)acc.ay.value());
-        fz[i] = (float)(-c.e24 * (double)acc.az.value());
+        stream_store(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
+        stream_store(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
+        stream_store(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
         const double ei = c.e4 * (double)acc.ae.value();
-        if (e) e[i] = (float)ei;
+        if (e) stream_store(&e[i], (float)ei);
         esum = ei;
         vsum = c.e24 * (double)acc.av.value();
     }

@@ -54,14 +54,6 @@ __device__ inline void cell_coords(const GridGeom& g, uint32_t key, int32_t& cx,
     cz = (int32_t)(r / (uint32_t)g.g[1]);
 }
 
-// Lower bound of the distance along one axis from coordinate v to cell cc.
-__device__ inline float axis_gap(float v, int32_t cc, int32_t home, float lo, float cw, float margin) {
-    float gap = 0.f;
-    if (cc > home) gap = (lo + (float)cc * cw) - v;
-    else if (cc < home) gap = v - (lo + (float)(cc + 1) * cw);
-    return fmaxf(gap - margin, 0.f);
-}
-
 // Particles in cell c of the cell_edge grid cg, from the offsets foff of the refined
 // grid fg (edge cg.cw / Kx along x, cg.cw / K along y and z).  For a power of two K,
 // x / (cw/K) is exactly K (x / cw) in binary floating point and floor(floor(K q) / K) =
@@ -165,15 +157,37 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
     // a row are those of qx -+ w inv_cw, truncated (a negative lower bound truncates to
     // 0, like the clamp of the cell rule) and clipped to [x0, x1]
     const float qx = (pi.x - g.lo[0]) * tr.inv_cw;
-    // Slot range [b, en) of the trimmed x-cells of row (zz, yy); empty outside the
-    // clipped stencil or beyond the cutoff.
-    auto row_range = [&](int32_t zz, int32_t yy, uint32_t& b, uint32_t& en) {
+    // Per particle, once: its distances to the faces of its own cell in y and z, and
+    // its own row.  The gap to the row d cells away is then |d| - 1 cell edges plus the
+    // distance to the near face (one FMA), and a row's index is the own row plus a
+    // stencil-uniform offset; both were per-row arithmetic on the cell coordinates.
+    // (The FMA rounds differently from the direct lo + c cw - v, by a few ulp of the
+    // coordinates, far inside the margin.)
+    float fy_lo = 0.f, fy_hi = 0.f, fz_lo = 0.f, fz_hi = 0.f;
+    if (trim) {
+        fy_lo = pi.y - (g.lo[1] + (float)cy * g.cw[1]);
+        fy_hi = (g.lo[1] + (float)(cy + 1) * g.cw[1]) - pi.y;
+        fz_lo = pi.z - (g.lo[2] + (float)cz * g.cw[2]);
+        fz_hi = (g.lo[2] + (float)(cz + 1) * g.cw[2]) - pi.z;
+    }
+    const uint32_t home = cbase + (uint32_t)((cz * g.g[1] + cy) * g.g[0]);
+    // the clipped stencil as offsets from the particle's cell
+    const int32_t dz0 = z0 - cz, dz1 = z1 - cz, dy0 = y0 - cy, dy1 = y1 - cy;
+    auto gap = [&](int32_t d, float near_lo, float near_hi, float cw) {
+        float v = 0.f;
+        if (d > 0) v = __builtin_fmaf((float)(d - 1), cw, near_hi);
+        else if (d < 0) v = __builtin_fmaf((float)(-d - 1), cw, near_lo);
+        return fmaxf(v - tr.margin, 0.f);
+    };
+    // Slot range [b, en) of the trimmed x-cells of row (cz + dz, cy + dy); empty
+    // outside the clipped stencil or beyond the cutoff.
+    auto row_range = [&](int32_t dz, int32_t dy, uint32_t& b, uint32_t& en) {
         b = en = 0u;
-        if (zz < z0 || zz > z1 || yy < y0 || yy > y1) return;
+        if (dz < dz0 || dz > dz1 || dy < dy0 || dy > dy1) return;
         int32_t xa = x0, xb = x1;
         if (trim) {
-            const float gz = axis_gap(pi.z, zz, cz, g.lo[2], g.cw[2], tr.margin);
-            const float gy = axis_gap(pi.y, yy, cy, g.lo[1], g.cw[1], tr.margin);
+            const float gz = gap(dz, fz_lo, fz_hi, g.cw[2]);
+            const float gy = gap(dy, fy_lo, fy_hi, g.cw[1]);
             const float w2 = tr.r2 - gz * gz - gy * gy;
             if (w2 < 0.f) return;
             // hardware square root (about 1 ulp, no correctly rounded fix-up): its
@@ -182,7 +196,7 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
             xa = min(max(x0, (int32_t)__builtin_fmaf(-w, tr.inv_cw, qx)), x1);
             xb = min(x1, (int32_t)__builtin_fmaf(w, tr.inv_cw, qx));
         }
-        const uint32_t row = cbase + (uint32_t)((zz * g.g[1] + yy) * g.g[0]);
+        const uint32_t row = home + (uint32_t)(dz * g.g[1] + dy) * (uint32_t)g.g[0];
         b = off[row + xa];
         en = off[row + xb + 1];
     };
@@ -197,9 +211,9 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
                 while (__popcll(__ballot(pc > 0)) >= (unsigned)kDrainLanes) drain_one();
             }
             uint32_t t, en, b2, e2;
-            row_range(cz + dz, cy + dy, t, en);
+            row_range(dz, dy, t, en);
             if (dz == 0 && dy == 0) b2 = e2 = 0u;
-            else row_range(cz - dz, cy - dy, b2, e2);
+            else row_range(-dz, -dy, b2, e2);
             if (t >= en) {
                 t = b2;
                 en = e2;
