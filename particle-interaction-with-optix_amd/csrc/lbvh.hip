@@ -228,11 +228,19 @@ __global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ 
     box[2 * i + 1] = make_float4(bb.hi[0], bb.hi[1], bb.hi[2], __int_as_float(rope_by_end[r.y]));
 }
 
-// Does the query box [p - h, p + h] meet node bounds [lo, hi]?  Differences in FP32
-// are monotone in their operands, so the test never rejects a point within h.
+// Does the query box [p - h, p + h] meet node bounds [lo, hi]?  Per axis, the gap
+// max(lo - p, p - hi) is how far p lies outside [lo, hi] (negative inside); the boxes
+// meet when no gap exceeds h.  Differences in FP32 are monotone in their operands, so
+// the test never rejects a point within h.  For finite values this is exactly the six
+// one-sided tests; a NaN difference is dropped by fmaxf, so the test can only accept
+// more nodes, never fewer, and the leaf test decides.  The max form is 6 subtractions,
+// 2 three-way maxima and one compare, where six compares joined by && become a nest of
+// divergent branches.
 __device__ inline bool overlaps(float4 p, float h, float4 lo, float4 hi) {
-    return (lo.x - p.x) <= h && (p.x - hi.x) <= h && (lo.y - p.y) <= h && (p.y - hi.y) <= h && (lo.z - p.z) <= h &&
-           (p.z - hi.z) <= h;
+    const float gx = fmaxf(lo.x - p.x, p.x - hi.x);
+    const float gy = fmaxf(lo.y - p.y, p.y - hi.y);
+    const float gz = fmaxf(lo.z - p.z, p.z - hi.z);
+    return fmaxf(fmaxf(gx, gy), gz) <= h;
 }
 
 // Visits every leaf whose position lies within the query box of leaf k, calling
@@ -248,6 +256,14 @@ __device__ inline bool overlaps(float4 p, float h, float4 lo, float4 hi) {
 // well-formed tree never needs more, but it is checked).  list: this lane's entries, at list[q * stride].
 // The list is LDS (kList * kLbvhBlock ints per block), which bounds the waves per CU:
 // 16 entries and 64-thread blocks measured fastest (list 8..64, blocks 64..256).
+//
+// The loops are written for the SIMT machine: a wave runs every path any of its lanes
+// takes, so internal nodes and buckets share one step (load the two box halves, test,
+// pick the next node, list a bucket that meets the box) with no branch on the kind of
+// node, and a leaf test is three compares of |p_j - p| (fl(a - b) = -fl(b - a), so
+// |d| <= h is exactly the two one-sided tests of the reference's point-in-box) joined
+// by `&`.  Lone leaves (leaves outside every bucket: on C5, one step in a thousand)
+// keep a branch of their own.
 constexpr int kList = 16;
 constexpr int kLbvhBlock = 64;
 constexpr int kLeafGroup = 2;  // bucket leaves loaded and tested per step of the scan
@@ -259,12 +275,11 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
                                 float4 p, float h, uint32_t limit, unsigned long long& tests, int* __restrict__ list,
                                 uint32_t stride, F&& f) {
     if (n == 1) return true;
+    uint32_t in_box = 0;
     auto leaf_test = [&](int slot, const float4& pj) {
-        if ((pj.x - p.x) <= h && (p.x - pj.x) <= h && (pj.y - p.y) <= h && (p.y - pj.y) <= h &&
-            (pj.z - p.z) <= h && (p.z - pj.z) <= h) {
-            ++tests;
-            if (slot != k) f(slot, pj);
-        }
+        const bool in = (fabsf(pj.x - p.x) <= h) & (fabsf(pj.y - p.y) <= h) & (fabsf(pj.z - p.z) <= h);
+        in_box += in ? 1u : 0u;
+        if (in & (slot != k)) f(slot, pj);
     };
     int node = n <= kBucket ? (0 | kCollapsed) : 0;  // a small tree is one bucket
     uint32_t steps = 0;
@@ -278,138 +293,25 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
                 node = kSentinel;
                 break;
             }
-            if (node < 0) {  // bucket
-                const int b = node & 0x7fffffff;
-                const float4 lo = box[2 * b], hi = box[2 * b + 1];
-                if (overlaps(p, h, lo, hi)) list[(cnt++) * stride] = b;
-                node = __float_as_int(hi.w);
-            } else if (node >= n - 1) {  // a leaf that is not in a bucket
-                list[(cnt++) * stride] = node;
-                node = rope_by_end[node - (n - 1)];
-            } else {
-                const float4 lo = box[2 * node], hi = box[2 * node + 1];
-                node = overlaps(p, h, lo, hi) ? __float_as_int(lo.w) : __float_as_int(hi.w);
+            const int b = node & 0x7fffffff;
+            if (b >= n - 1) {  // a lone leaf
+                list[(cnt++) * stride] = b;
+                node = rope_by_end[b - (n - 1)];
+                continue;
             }
+            const float4 lo = box[2 * b], hi = box[2 * b + 1];
+            const bool ov = overlaps(p, h, lo, hi);
+            const bool bucket = node < 0;
+            if (ov & bucket) list[(cnt++) * stride] = b;
+            node = (ov & !bucket) ? __float_as_int(lo.w) : __float_as_int(hi.w);
         }
         // phase 2: a bucket's leaves in groups of kLeafGroup, loaded together (the leaf
         // array has spare entries past the last leaf, so the loads need no clamp)
         for (unsigned q = 0; q < cnt; ++q) {
             const int e = list[q * stride];
-            if (e >= n - 1) {
-                leaf_test(e - (n - 1), leaf[e - (n - 1)]);
-            } else {
-                const int2 r = range[e];
-                for (int slot = r.x; slot <= r.y; slot += kLeafGroup) {
-                    float4 pj[kLeafGroup];
-#pragma unroll
-                    for (int u = 0; u < kLeafGroup; ++u) pj[u] = leaf[slot + u];
-                    leaf_test(slot, pj[0]);
-#pragma unroll
-                    for (int u = 1; u < kLeafGroup; ++u)
-                        if (slot + u <= r.y) leaf_test(slot + u, pj[u]);
-                }
-            }
-        }
-    }
-    return ok;
-}
-
-// Group walks.  The queries of kGroup consecutive leaves (Morton order, so nearby
-// points) walk nearly the same nodes, and a walk is a chain of dependent node loads,
-// so each group walks once: with the group's bounding box grown by h it lists, in walk
-// (depth-first) order, every bucket and lone leaf that any of its queries can reach
-// (k_group_walk).  A query then takes its group's list instead of walking: it tests
-// each listed bucket's box against its own query box and scans the buckets that meet
-// it, and tests each listed lone leaf.  That is the same set of leaves in the same order
-// as its own walk (a leaf within the query box lies in every box above it), so the
-// pairs and the order of their terms, hence the bits, are those of traverse().  A group
-// whose list would exceed kGroupList entries is marked, and its queries walk
-// themselves.
-constexpr int kGroupBits = 4;
-constexpr int kGroup = 1 << kGroupBits;
-constexpr int kGroupList = 64;
-
-__device__ inline bool overlaps_box(float4 glo, float4 ghi, float h, float4 lo, float4 hi) {
-    return (lo.x - ghi.x) <= h && (glo.x - hi.x) <= h && (lo.y - ghi.y) <= h && (glo.y - hi.y) <= h &&
-           (lo.z - ghi.z) <= h && (glo.z - hi.z) <= h;
-}
-
-// One thread per group of kGroup leaves: the group's list (bucket nodes, or leaf node
-// ids >= n - 1) and its length, or kGroupList + 1 when it does not fit (or the walk
-// exceeds `limit` steps: its queries then walk, and trip the guard themselves).
-__global__ __launch_bounds__(256) void k_group_walk(const float4* __restrict__ leaf, const float4* __restrict__ box,
-                                                    const int* __restrict__ rope_by_end, int n, float h,
-                                                    uint32_t limit, int* __restrict__ glist,
-                                                    int* __restrict__ gcount) {
-    const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int first = g * kGroup;
-    if (first >= n) return;
-    const int last = first + kGroup < n ? first + kGroup : n;
-    float4 glo = leaf[first], ghi = glo;
-    for (int k = first + 1; k < last; ++k) {
-        const float4 p = leaf[k];
-        glo.x = fminf(glo.x, p.x);
-        glo.y = fminf(glo.y, p.y);
-        glo.z = fminf(glo.z, p.z);
-        ghi.x = fmaxf(ghi.x, p.x);
-        ghi.y = fmaxf(ghi.y, p.y);
-        ghi.z = fmaxf(ghi.z, p.z);
-    }
-    int* out = glist + (size_t)g * kGroupList;
-    int cnt = 0;
-    uint32_t steps = 0;
-    int node = 0;
-    while (node != kSentinel) {
-        if (++steps > limit || cnt > kGroupList) {
-            cnt = kGroupList + 1;
-            break;
-        }
-        if (node < 0) {  // bucket
-            const int b = node & 0x7fffffff;
-            const float4 lo = box[2 * b], hi = box[2 * b + 1];
-            if (overlaps_box(glo, ghi, h, lo, hi)) {
-                if (cnt < kGroupList) out[cnt] = b;
-                ++cnt;
-            }
-            node = __float_as_int(hi.w);
-        } else if (node >= n - 1) {  // a leaf that is not in a bucket
-            if (cnt < kGroupList) out[cnt] = node;
-            ++cnt;
-            node = rope_by_end[node - (n - 1)];
-        } else {
-            const float4 lo = box[2 * node], hi = box[2 * node + 1];
-            node = overlaps_box(glo, ghi, h, lo, hi) ? __float_as_int(lo.w) : __float_as_int(hi.w);
-        }
-    }
-    gcount[g] = cnt > kGroupList ? kGroupList + 1 : cnt;
-}
-
-// traverse() through the group's list when it has one (glist == nullptr: always walk).
-template <class F>
-__device__ inline bool traverse_grouped(const float4* __restrict__ leaf, const int2* __restrict__ range,
-                                        const float4* __restrict__ box, const int* __restrict__ rope_by_end, int n,
-                                        int k, float4 p, float h, uint32_t limit, unsigned long long& tests,
-                                        int* __restrict__ list, uint32_t stride, const int* __restrict__ glist,
-                                        const int* __restrict__ gcount, F&& f) {
-    const int g = k >> kGroupBits;
-    const int cnt = glist ? gcount[g] : kGroupList + 1;
-    if (cnt > kGroupList) return traverse(leaf, range, box, rope_by_end, n, k, p, h, limit, tests, list, stride, f);
-    auto leaf_test = [&](int slot, const float4& pj) {
-        if ((pj.x - p.x) <= h && (p.x - pj.x) <= h && (pj.y - p.y) <= h && (p.y - pj.y) <= h &&
-            (pj.z - p.z) <= h && (p.z - pj.z) <= h) {
-            ++tests;
-            if (slot != k) f(slot, pj);
-        }
-    };
-    const int* entries = glist + (size_t)g * kGroupList;
-    for (int q = 0; q < cnt; ++q) {
-        const int e = entries[q];
-        if (e >= n - 1) {
-            leaf_test(e - (n - 1), leaf[e - (n - 1)]);
-        } else {
-            const float4 lo = box[2 * e], hi = box[2 * e + 1];
-            if (!overlaps(p, h, lo, hi)) continue;
-            const int2 r = range[e];
+            int2 r;
+            if (e >= n - 1) r = make_int2(e - (n - 1), e - (n - 1));
+            else r = range[e];
             for (int slot = r.x; slot <= r.y; slot += kLeafGroup) {
                 float4 pj[kLeafGroup];
 #pragma unroll
@@ -421,7 +323,8 @@ __device__ inline bool traverse_grouped(const float4* __restrict__ leaf, const i
             }
         }
     }
-    return true;
+    tests += in_box;
+    return ok;
 }
 
 template <int PRED, int PREC, bool VIR>
@@ -430,8 +333,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
                                                     float* __restrict__ fx, float* __restrict__ fy,
                                                     float* __restrict__ fz, float* __restrict__ e,
                                                     double* __restrict__ partials, lj_result* __restrict__ res,
-                                                    int* __restrict__ overflow, const int* __restrict__ glist,
-                                                    const int* __restrict__ gcount) {
+                                                    int* __restrict__ overflow) {
     extern __shared__ int lds_list[];
     int* list = lds_list + threadIdx.x;
     // consecutive queries walk nearly the same nodes: keep them on one XCD (own L2)
@@ -444,8 +346,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
         const float4 pi = leaf[k];
         acc.set_i(pi);
         acc.allow_exact_diff(c, pi, true);  // over the lanes with a query
-        const bool ok = traverse_grouped(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x,
-                                         glist, gcount, [&](int, float4 pj) {
+        const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x, [&](int, float4 pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float d2 = dist2(dx, dy, dz);
             const int mult = pair_mult<PRED>(c, pi, pj, d2);
@@ -477,8 +378,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
 template <int PRED>
 __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_count(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                                     const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, uint32_t limit, PairConst c,
-                                                    uint32_t* __restrict__ rowcnt, int* __restrict__ overflow,
-                                                    const int* __restrict__ glist, const int* __restrict__ gcount) {
+                                                    uint32_t* __restrict__ rowcnt, int* __restrict__ overflow) {
     extern __shared__ int lds_list[];
     int* list = lds_list + threadIdx.x;
     const int k = (int)(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
@@ -486,8 +386,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_count(const float4* __restr
     const float4 pi = leaf[k];
     uint32_t cnt = 0;
     unsigned long long tests = 0;
-    const bool ok = traverse_grouped(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x, glist,
-                                     gcount, [&](int, float4 pj) {
+    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x, [&](int, float4 pj) {
         if (pair_mult<PRED>(c, pi, pj, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z)) > 0) ++cnt;
     });
     if (!ok) atomicOr(overflow, 1);
@@ -498,8 +397,7 @@ template <int PRED>
 __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_fill(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                                    const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, uint32_t limit, PairConst c,
                                                    const unsigned long long* __restrict__ rows,
-                                                   uint32_t* __restrict__ cols, int* __restrict__ overflow,
-                                                   const int* __restrict__ glist, const int* __restrict__ gcount) {
+                                                   uint32_t* __restrict__ cols, int* __restrict__ overflow) {
     extern __shared__ int lds_list[];
     int* list = lds_list + threadIdx.x;
     const int k = (int)(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
@@ -509,8 +407,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_fill(const float4* __restri
     const unsigned long long r0 = rows[i], r1 = rows[i + 1];
     unsigned long long w = r0;
     unsigned long long tests = 0;
-    const bool ok = traverse_grouped(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x, glist,
-                                     gcount, [&](int, float4 pj) {
+    const bool ok = traverse(leaf, range, box, rope_by_end, n, k, pi, h, limit, tests, list, blockDim.x, [&](int, float4 pj) {
         if (w < r1 && pair_mult<PRED>(c, pi, pj, dist2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z)) > 0)
             cols[w++] = __float_as_uint(pj.w);
     });
@@ -609,24 +506,6 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     return LJ_OK;
 }
 
-// Lists of the leaf groups (k_group_walk) for the walks of this call; nullptr when the
-// tree is too small to be worth it (the queries then walk).
-static int group_lists(lj_context* ctx, int64_t n, float h, uint32_t lim, const int** glist, const int** gcount,
-                       hipStream_t s) {
-    *glist = *gcount = nullptr;
-    if (n <= 2 * kBucket) return LJ_OK;
-    LbvhWork& L = ctx->lbvh;
-    const size_t groups = (size_t)div_up(n, kGroup);
-    LJ_HIP(ctx, L.glist.grow(groups * kGroupList * sizeof(int)));
-    LJ_HIP(ctx, L.gcount.grow(groups * sizeof(int)));
-    hipLaunchKernelGGL(k_group_walk, dim3(div_up((int64_t)groups, 256)), dim3(256), 0, s,
-                       (const float4*)ctx->bin.pos.as<float4>(), (const float4*)L.node_box.as<float4>(),
-                       (const int*)L.rope.as<int>(), (int)n, h, lim, L.glist.as<int>(), L.gcount.as<int>());
-    *glist = L.glist.as<int>();
-    *gcount = L.gcount.as<int>();
-    return LJ_OK;
-}
-
 // Query half-width.  Every pair a cutoff predicate accepts has |d_axis| <= rc (1 +
 // 2^-20); the TRIANGLE ray model measures the cutoff to the centre rebuilt from the
 // panel corners, which can lie a few ulp of the coordinates off the particle, so its
@@ -690,7 +569,7 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
 #define LJ_LB(PRED, PREC, VIR)                                                                                        \
     hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, fx, \
-                       fy, fz, e, partials, res, ovf, glist, gcount)
+                       fy, fz, e, partials, res, ovf)
 #define LJ_LB_PREC(PRED)                                                                                    \
     switch (p->precision) {                                                                                 \
     case LJ_PREC_MIXED: if (vir) LJ_LB(PRED, LJ_PREC_MIXED, true); else LJ_LB(PRED, LJ_PREC_MIXED, false); break; \
@@ -698,8 +577,6 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     default: if (vir) LJ_LB(PRED, LJ_PREC_FP32_COMP, true); else LJ_LB(PRED, LJ_PREC_FP32_COMP, false); break;    \
     }
     PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
-    const int *glist, *gcount;
-    LJ_TRY(group_lists(ctx, n, h, lim, &glist, &gcount, s));
     if (n > 0) {
         switch (p->predicate) {
         case LJ_PRED_AABB: LJ_LB_PREC(LJ_PRED_AABB); break;
@@ -709,7 +586,7 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
         }
     } else {
         hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true>), dim3(1), dim3(kLbvhBlock), lds, s, leaf, range, rope, box,
-                           0, h, lim, c, fx, fy, fz, e, partials, res, ovf, glist, gcount);
+                           0, h, lim, c, fx, fy, fz, e, partials, res, ovf);
     }
 #undef LJ_LB_PREC
 #undef LJ_LB
@@ -743,11 +620,9 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     const int2* range = ctx->lbvh.range.as<int2>();
     const int* rope = ctx->lbvh.rope.as<int>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
-    const int *glist, *gcount;
-    LJ_TRY(group_lists(ctx, n, h, lim, &glist, &gcount, s));
     if (n > 0) {
         switch (p->predicate) {
-#define LJ_C(P) case P: hipLaunchKernelGGL(k_lbvh_count<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, rowcnt, ovf, glist, gcount); break;
+#define LJ_C(P) case P: hipLaunchKernelGGL(k_lbvh_count<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, rowcnt, ovf); break;
             LJ_C(LJ_PRED_AABB) LJ_C(LJ_PRED_CUSTOM) LJ_C(LJ_PRED_SPHERE) LJ_C(LJ_PRED_TRIANGLE)
 #undef LJ_C
         }
@@ -763,7 +638,7 @@ int lj_lbvh_pairs(lj_context* ctx, const float* x, const float* y, const float* 
     if (total > capacity) return fail(ctx, LJ_ERR_CAPACITY, "pair list needs %llu entries, capacity is %llu", total, capacity);
     if (total == 0) return LJ_OK;
     switch (p->predicate) {
-#define LJ_F(P) case P: hipLaunchKernelGGL(k_lbvh_fill<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, (const unsigned long long*)row_offsets, cols, ovf, glist, gcount); break;
+#define LJ_F(P) case P: hipLaunchKernelGGL(k_lbvh_fill<P>, dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, (const unsigned long long*)row_offsets, cols, ovf); break;
         LJ_F(LJ_PRED_AABB) LJ_F(LJ_PRED_CUSTOM) LJ_F(LJ_PRED_SPHERE) LJ_F(LJ_PRED_TRIANGLE)
 #undef LJ_F
     }

@@ -115,8 +115,6 @@ struct LbvhWork {
     DevBuf chunks;        // float4 x2 bounds of aligned leaf chunks (levels of 4, 16, ..., 2^24 leaves)
     DevBuf rope;          // int per leaf: the node after every subtree ending at that leaf
     DevBuf flags;         // the traversal overflow flag
-    DevBuf glist;         // int per entry: each leaf group's walk list (buckets and lone leaves)
-    DevBuf gcount;        // int per leaf group: its list length (or the full mark)
 };
 
 struct BatchWork {
