@@ -22,7 +22,7 @@ void for_each_buffer(lj_context* ctx, F&& f) {
                       &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets, &b.cand})
         f(*d);
     LbvhWork& l = ctx->lbvh;
-    for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags}) f(*d);
+    for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags, &l.bucket_root}) f(*d);
     BatchWork& t = ctx->batch;
     for (DevBuf* d : {&t.sys, &t.hints, &t.npair, &t.ev, &t.wrec}) f(*d);
     f(ctx->tune_out);

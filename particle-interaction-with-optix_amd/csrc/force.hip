@@ -19,9 +19,9 @@ namespace lj {
 
 namespace {
 
-// One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues in
-// dynamic LDS (depth x block entries of 12 bytes, one block of rings per wave; none
-// without QUEUE).  Outputs go to the caller's
+// One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues
+// (QUEUE: depth x block entries of 16 bytes, one block of rings per wave) or row tables
+// (kRowTable x block entries of 8 bytes) in dynamic LDS.  Outputs go to the caller's
 // order (pi.w holds the original index); per-block FP64 energy and virial partials are
 // reduced later in a fixed order.
 template <int PRED, int PREC, bool VIR, bool QUEUE>
@@ -40,9 +40,7 @@ __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict
     acc.set_i(pi);
     acc.allow_exact_diff(c, pi, active);
     const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u,
-                                                              Rx, R, tr, c,
-                                                              queue + (threadIdx.x / kWaveSize) * depth * kWaveSize +
-                                                                  __lane_id(),
+                                                              Rx, R, tr, c, wave_lds_block<QUEUE>(queue, depth),
                                                               depth, acc);
     double esum = 0.0, vsum = 0.0;
     if (active) {

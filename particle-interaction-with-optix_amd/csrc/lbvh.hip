@@ -84,18 +84,23 @@ __device__ inline int delta(const uint32_t* __restrict__ code, int n, int i, int
 // Node numbering: internal nodes 0..n-2 (0 is the root), leaf k is node n-1+k (n <=
 // 2^30, so every id fits 31 bits).  A child or rope reference with bit 31 set is a
 // bucket: an internal node covering at most kBucket leaves, scanned instead of walked.
-// Measured on 4,194,304 clustered particles, rc = 0.005 (per step): 8 leaves 5.1 ms, 16
-// leaves 4.6 ms with one leaf scanned at a time; with leaves scanned in pairs 12: 4.34,
-// 16: 4.20, 24: 4.15, 32: 4.16 ms.
-constexpr int kBucket = 24;
+// Measured on 4,194,304 clustered particles, rc = 0.005, traversal + forces per step,
+// buckets of B leaves scanned G at a time: B/G 8/2 2.82 ms, 12/2 2.66, 16/2 2.61, 24/2
+// 2.65, 16/1 2.75, 12/4 2.61, 16/4 2.56, 24/4 2.61, 32/4 2.68.  Smaller buckets scan
+// fewer leaves outside the query box (16: 173 leaves per query for 53 inside it) and
+// walk more nodes (16: 111 steps per query, 24: 97).
+constexpr int kBucket = 16;
 constexpr int kCollapsed = (int)0x80000000u;
 constexpr int kSentinel = 0x7fffffff;  // the rope of the last subtree: end of the walk
 
 // Karras' construction.  Every split position gamma in [0, n-2] belongs to exactly
 // one internal node, whose right child starts at leaf gamma + 1; that right child is
 // therefore the rope of every subtree that ends at leaf gamma: rope_by_end[gamma].
+// Every internal node but the root is the child of exactly one node, which writes its
+// bucket-root mark, so the marks need no clearing.
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ code, int n, int2* __restrict__ child,
-                                                int2* __restrict__ range, int* __restrict__ rope_by_end) {
+                                                int2* __restrict__ range, int* __restrict__ rope_by_end,
+                                                uint8_t* __restrict__ bucket_root) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (delta(code, n, i, i + 1) - delta(code, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -117,8 +122,16 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     const int lo = i < j ? i : j, hi = i < j ? j : i;
     int left = (lo == gamma) ? (n - 1 + gamma) : gamma;
     int right = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
-    if (left < n - 1 && gamma - lo + 1 <= kBucket) left |= kCollapsed;
-    if (right < n - 1 && hi - gamma <= kBucket) right |= kCollapsed;
+    if (left < n - 1) {
+        const bool b = gamma - lo + 1 <= kBucket && hi - lo + 1 > kBucket;
+        bucket_root[left] = b;
+        if (gamma - lo + 1 <= kBucket) left |= kCollapsed;
+    }
+    if (right < n - 1) {
+        const bool b = hi - gamma <= kBucket && hi - lo + 1 > kBucket;
+        bucket_root[right] = b;
+        if (hi - gamma <= kBucket) right |= kCollapsed;
+    }
     child[i] = make_int2(left, right);
     range[i] = make_int2(lo, hi);
     rope_by_end[gamma] = right;
@@ -189,10 +202,14 @@ __global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ 
                                                      const int2* __restrict__ range, const int2* __restrict__ child,
                                                      const int* __restrict__ rope_by_end,
                                                      const float4* __restrict__ chunks, ChunkLevels lv,
+                                                     const uint8_t* __restrict__ bucket_root,
                                                      float4* __restrict__ box) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int2 r = range[i];
+    // A walk reads the box of the root, of nodes above kBucket leaves and of bucket
+    // roots, never of a node inside a bucket (about 7 in 8 nodes): those keep no box.
+    if (i != 0 && r.y - r.x + 1 <= kBucket && !bucket_root[i]) return;
     uint32_t a = (uint32_t)r.x;
     const uint32_t b = (uint32_t)r.y + 1u;  // exclusive, <= n
     Bounds bb;
@@ -266,7 +283,7 @@ __device__ inline bool overlaps(float4 p, float h, float4 lo, float4 hi) {
 // keep a branch of their own.
 constexpr int kList = 16;
 constexpr int kLbvhBlock = 64;
-constexpr int kLeafGroup = 2;  // bucket leaves loaded and tested per step of the scan
+constexpr int kLeafGroup = 4;  // bucket leaves loaded and tested per step of the scan
 static_assert(kLeafGroup - 1 <= kPosSpare, "unclamped leaf loads need spare entries");
 
 template <class F>
@@ -452,6 +469,7 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     BinWork& B = ctx->bin;
     const size_t nn = (size_t)(n > 0 ? n : 1);
     LJ_HIP(ctx, L.node_child.grow(nn * sizeof(int2)));
+    LJ_HIP(ctx, L.bucket_root.grow(nn));
     LJ_HIP(ctx, L.range.grow(nn * sizeof(int2)));
     LJ_HIP(ctx, L.rope.grow(nn * sizeof(int)));
     LJ_HIP(ctx, L.flags.grow(64));
@@ -483,7 +501,7 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, rope + (n - 1), 1u, kSentinel);
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, s, code, (int)n, L.node_child.as<int2>(),
-                           range, rope);
+                           range, rope, L.bucket_root.as<uint8_t>());
     // chunk bounds, level by level (level 0 from the leaves, level l + 1 from level l)
     const float4* leaves = B.pos.as<float4>();
     {
@@ -501,7 +519,8 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     if (n > 1)
         hipLaunchKernelGGL(k_node_bounds, dim3(div_up(n - 1, 256)), dim3(256), 0, s, leaves, (int)n,
                            (const int2*)range, (const int2*)L.node_child.as<int2>(), (const int*)rope,
-                           (const float4*)L.chunks.as<float4>(), lv, L.node_box.as<float4>());
+                           (const float4*)L.chunks.as<float4>(), lv, (const uint8_t*)L.bucket_root.as<uint8_t>(),
+                           L.node_box.as<float4>());
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;
 }

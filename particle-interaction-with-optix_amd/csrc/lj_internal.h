@@ -86,7 +86,7 @@ struct Status {
 
 // Workspace of one binned system (or of a concatenated batch).
 // Entries past the last sorted particle in BinWork::pos (never written, read but unused).
-constexpr int kPosSpare = 2;
+constexpr int kPosSpare = 3;
 
 struct BinWork {
     DevBuf keys[2];      // u32 radix keys (ping-pong)
@@ -115,6 +115,7 @@ struct LbvhWork {
     DevBuf chunks;        // float4 x2 bounds of aligned leaf chunks (levels of 4, 16, ..., 2^24 leaves)
     DevBuf rope;          // int per leaf: the node after every subtree ending at that leaf
     DevBuf flags;         // the traversal overflow flag
+    DevBuf bucket_root;   // uint8 per internal node: 1 when its parent scans it as a bucket
 };
 
 struct BatchWork {
