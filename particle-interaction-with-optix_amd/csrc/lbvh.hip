@@ -96,8 +96,11 @@ constexpr int kSentinel = 0x7fffffff;  // the rope of the last subtree: end of t
 // Karras' construction.  Every split position gamma in [0, n-2] belongs to exactly
 // one internal node, whose right child starts at leaf gamma + 1; that right child is
 // therefore the rope of every subtree that ends at leaf gamma: rope_by_end[gamma].
-// Every internal node but the root is the child of exactly one node, which writes its
-// bucket-root mark, so the marks need no clearing.
+// A walk reads the box of the root, of the nodes above kBucket leaves and of bucket
+// roots, never of a node inside a bucket (about 7 in 8 nodes).  Every internal node but
+// the root is the child of exactly one node, which marks it when it is a bucket root
+// (so the marks need no clearing); k_node_bounds computes the boxes of the marked, the
+// large and the root nodes only.
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ code, int n, int2* __restrict__ child,
                                                 int2* __restrict__ range, int* __restrict__ rope_by_end,
                                                 uint8_t* __restrict__ bucket_root) {
@@ -122,14 +125,13 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     const int lo = i < j ? i : j, hi = i < j ? j : i;
     int left = (lo == gamma) ? (n - 1 + gamma) : gamma;
     int right = (hi == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+    const bool big = hi - lo + 1 > kBucket;
     if (left < n - 1) {
-        const bool b = gamma - lo + 1 <= kBucket && hi - lo + 1 > kBucket;
-        bucket_root[left] = b;
+        bucket_root[left] = big && gamma - lo + 1 <= kBucket;
         if (gamma - lo + 1 <= kBucket) left |= kCollapsed;
     }
     if (right < n - 1) {
-        const bool b = hi - gamma <= kBucket && hi - lo + 1 > kBucket;
-        bucket_root[right] = b;
+        bucket_root[right] = big && hi - gamma <= kBucket;
         if (hi - gamma <= kBucket) right |= kCollapsed;
     }
     child[i] = make_int2(left, right);
@@ -198,18 +200,11 @@ __global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__
 // kChunk - 1 elements per level on each side; narrow chunks keep that short).  Level l
 // (0 .. kLevels-1, chunks of kChunk^(l + 1) leaves) starts at pair lv.at[l] of chunks.
 // The .w lanes carry the walk: lo.w = the left child, hi.w = the rope (int bits).
-__global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ leaf, int n,
-                                                     const int2* __restrict__ range, const int2* __restrict__ child,
-                                                     const int* __restrict__ rope_by_end,
-                                                     const float4* __restrict__ chunks, ChunkLevels lv,
-                                                     const uint8_t* __restrict__ bucket_root,
-                                                     float4* __restrict__ box) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
+__device__ inline void node_box(const float4* __restrict__ leaf, const int2* __restrict__ range,
+                                const int2* __restrict__ child, const int* __restrict__ rope_by_end,
+                                const float4* __restrict__ chunks, const ChunkLevels& lv, int i,
+                                float4* __restrict__ box) {
     const int2 r = range[i];
-    // A walk reads the box of the root, of nodes above kBucket leaves and of bucket
-    // roots, never of a node inside a bucket (about 7 in 8 nodes): those keep no box.
-    if (i != 0 && r.y - r.x + 1 <= kBucket && !bucket_root[i]) return;
     uint32_t a = (uint32_t)r.x;
     const uint32_t b = (uint32_t)r.y + 1u;  // exclusive, <= n
     Bounds bb;
@@ -243,6 +238,38 @@ __global__ __launch_bounds__(256) void k_node_bounds(const float4* __restrict__ 
     }
     box[2 * i] = make_float4(bb.lo[0], bb.lo[1], bb.lo[2], __int_as_float(child[i].x));
     box[2 * i + 1] = make_float4(bb.hi[0], bb.hi[1], bb.hi[2], __int_as_float(rope_by_end[r.y]));
+}
+
+// The boxes of the nodes a walk reads.  Each wave takes kNodesPerWave consecutive
+// nodes, lists in LDS those whose box a walk reads, and computes them 64 at a time, so
+// its lanes stay busy although only about one node in eight needs a box (in node order
+// about as many lanes would idle).
+constexpr int kNodeBoundsBlock = 256;
+constexpr int kNodesPerWave = 512;
+__global__ __launch_bounds__(kNodeBoundsBlock) void k_node_bounds(const float4* __restrict__ leaf, int n,
+                                                     const int2* __restrict__ range, const int2* __restrict__ child,
+                                                     const int* __restrict__ rope_by_end,
+                                                     const float4* __restrict__ chunks, ChunkLevels lv,
+                                                     const uint8_t* __restrict__ bucket_root,
+                                                     float4* __restrict__ box) {
+    __shared__ int todo[kNodeBoundsBlock / kWaveSize][kNodesPerWave];
+    int* const list = todo[threadIdx.x / kWaveSize];
+    const int first = (int)((blockIdx.x * (kNodeBoundsBlock / kWaveSize) + threadIdx.x / kWaveSize) * kNodesPerWave);
+    unsigned cnt = 0;  // wave-uniform
+#pragma unroll
+    for (int u = 0; u < kNodesPerWave / kWaveSize; ++u) {
+        const int i = first + u * kWaveSize + (int)__lane_id();
+        bool need = false;
+        if (i < n - 1) {
+            const int2 r = range[i];
+            need = i == 0 || r.y - r.x + 1 > kBucket || bucket_root[i];
+        }
+        const uint64_t m = __ballot(need);
+        if (need) list[cnt + sortk::rank_below(m)] = i;
+        cnt += (unsigned)__popcll(m);
+    }
+    for (unsigned j = __lane_id(); j < cnt; j += kWaveSize)
+        node_box(leaf, range, child, rope_by_end, chunks, lv, list[j], box);
 }
 
 // Does the query box [p - h, p + h] meet node bounds [lo, hi]?  Per axis, the gap
@@ -517,7 +544,8 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
         }
     }
     if (n > 1)
-        hipLaunchKernelGGL(k_node_bounds, dim3(div_up(n - 1, 256)), dim3(256), 0, s, leaves, (int)n,
+        hipLaunchKernelGGL(k_node_bounds, dim3(div_up(n - 1, kNodeBoundsBlock / kWaveSize * kNodesPerWave)),
+                           dim3(kNodeBoundsBlock), 0, s, leaves, (int)n,
                            (const int2*)range, (const int2*)L.node_child.as<int2>(), (const int*)rope,
                            (const float4*)L.chunks.as<float4>(), lv, (const uint8_t*)L.bucket_root.as<uint8_t>(),
                            L.node_box.as<float4>());
