@@ -148,6 +148,7 @@ constexpr int kChunkBits = 2;  // a chunk of level l + 1 holds 4 chunks of level
 constexpr int kChunk = 1 << kChunkBits;
 constexpr int kLevels = 24 / kChunkBits;  // chunks of 4, 16, ..., 2^24 leaves
 static_assert(kLevels * kChunkBits == 24, "the largest chunk holds 2^24 leaves");
+static_assert(kChunk == 4 && kLevels % 4 == 0, "k_chunk_bounds builds four levels of 4-way chunks per launch");
 
 // First chunk pair of each level in the chunk array (level l holds ceil(n / 4^(l+1))).
 struct ChunkLevels {
@@ -170,11 +171,19 @@ struct Bounds {
     }
 };
 
-// Bounds of aligned chunks: chunk c of the output covers inputs [kChunk c, kChunk c +
-// kChunk) (leaf positions when `leaves`, else (lo, hi) pairs of the level below);
-// kChunk lanes per chunk.
+// Bounds of aligned chunks, four levels per launch: a block of 256 threads takes 256
+// consecutive inputs (leaf positions when `leaves`, else (lo, hi) pairs of the level
+// below) and writes the chunks of 4, 16, 64 and 256 of them that it holds to out[0..3]
+// (level L + j has cnt[j] chunks; a partial last chunk takes what exists).  Groups of
+// 4, 16 and 64 lanes reduce with shuffles, the block's four waves through LDS.
+struct ChunkOut {
+    float4* out[4];
+    uint32_t cnt[4];
+};
+
 __global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__ in, bool leaves, uint32_t count,
-                                                      uint32_t nchunks, float4* __restrict__ out) {
+                                                      ChunkOut o) {
+    __shared__ float4 wlo[4], whi[4];
     const uint32_t e = blockIdx.x * 256u + threadIdx.x;  // input element
     Bounds b;
     b.empty();
@@ -182,16 +191,34 @@ __global__ __launch_bounds__(256) void k_chunk_bounds(const float4* __restrict__
         if (leaves) b.add(in[e], in[e]);
         else b.add(in[2 * e], in[2 * e + 1]);
     }
-#pragma unroll
-    for (int d = kChunk / 2; d > 0; d >>= 1)
-        for (int a = 0; a < 3; ++a) {
-            b.lo[a] = fminf(b.lo[a], __shfl_xor(b.lo[a], d, kChunk));
-            b.hi[a] = fmaxf(b.hi[a], __shfl_xor(b.hi[a], d, kChunk));
+    auto reduce = [&](int from, int to) {  // xor distances from .. to (powers of two)
+        for (int d = from; d >= to; d >>= 1)
+            for (int a = 0; a < 3; ++a) {
+                b.lo[a] = fminf(b.lo[a], __shfl_xor(b.lo[a], d, kWaveSize));
+                b.hi[a] = fmaxf(b.hi[a], __shfl_xor(b.hi[a], d, kWaveSize));
+            }
+    };
+    auto put = [&](int j, uint32_t c) {
+        if (c < o.cnt[j]) {
+            o.out[j][2 * c] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.f);
+            o.out[j][2 * c + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.f);
         }
-    const uint32_t c = e / kChunk;
-    if (e % kChunk == 0 && c < nchunks) {
-        out[2 * c] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.f);
-        out[2 * c + 1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.f);
+    };
+    reduce(2, 1);
+    if (e % 4 == 0) put(0, e / 4);
+    reduce(8, 4);
+    if (e % 16 == 0) put(1, e / 16);
+    reduce(32, 16);
+    if (e % 64 == 0) {
+        put(2, e / 64);
+        wlo[threadIdx.x / 64] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.f);
+        whi[threadIdx.x / 64] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.f);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        b.empty();
+        for (int w = 0; w < 4; ++w) b.add(wlo[w], whi[w]);
+        put(3, blockIdx.x);
     }
 }
 
@@ -529,18 +556,24 @@ static int lbvh_build(lj_context* ctx, const float* x, const float* y, const flo
     if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, 256)), dim3(256), 0, s, code, (int)n, L.node_child.as<int2>(),
                            range, rope, L.bucket_root.as<uint8_t>());
-    // chunk bounds, level by level (level 0 from the leaves, level l + 1 from level l)
+    // chunk bounds, four levels per launch (levels 0-3 from the leaves, 4-7 from level 3,
+    // 8-11 from level 7)
     const float4* leaves = B.pos.as<float4>();
     {
         uint64_t cnt = (uint64_t)n;
         const float4* in = leaves;
-        for (int l = 0; l < kLevels; ++l) {
-            const uint64_t nch = div_up(cnt, kChunk);
-            float4* out = L.chunks.as<float4>() + 2 * (size_t)lv.at[l];
+        for (int l = 0; l < kLevels; l += 4) {
+            ChunkOut o;
+            uint64_t c = cnt;
+            for (int j = 0; j < 4; ++j) {
+                c = div_up(c, kChunk);
+                o.out[j] = L.chunks.as<float4>() + 2 * (size_t)lv.at[l + j];
+                o.cnt[j] = (uint32_t)c;
+            }
             hipLaunchKernelGGL(k_chunk_bounds, dim3((unsigned)div_up(cnt, 256)), dim3(256), 0, s, in, l == 0,
-                               (uint32_t)cnt, (uint32_t)nch, out);
-            in = out;
-            cnt = nch;
+                               (uint32_t)cnt, o);
+            in = o.out[3];
+            cnt = o.cnt[3];
         }
     }
     if (n > 1)
