@@ -144,7 +144,7 @@ constexpr uint32_t kSharedWave = 0xffffffffu;
 template <int PRED, int PREC, bool QUEUE>
 __device__ inline void batch_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
                                    bool active, uint32_t key, const SysDesc& d, int Rx, int R, uint32_t depth,
-                                   void* __restrict__ wave_lds, float* __restrict__ fx,
+                                   QEntry* __restrict__ queue, float* __restrict__ fx,
                                    float* __restrict__ fy, float* __restrict__ fz, float* __restrict__ e, double& ei,
                                    double& vi, uint32_t& np) {
     const GridGeom g = d.g;
@@ -155,16 +155,16 @@ __device__ inline void batch_pairs(const float4* __restrict__ pos, const uint32_
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
     acc.set_i(pi);
     acc.allow_exact_diff(c, pi, active);
-    np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, Rx, R, tr, c,
-                                                wave_lds, depth, acc);
+    np = stencil_pairs<PRED, PREC, true, QUEUE>(pos, off, k, active, pi, g, base, key - base, Rx, R, tr, c, queue,
+                                                depth, acc);
     ei = vi = 0.0;
     if (active) {
         const uint32_t i = __float_as_uint(pi.w);
-        stream_store(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
-        stream_store(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
-        stream_store(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
+        fx[i] = (float)(-c.e24 * (double)acc.ax.value());
+        fy[i] = (float)(-c.e24 * (double)acc.ay.value());
+        fz[i] = (float)(-c.e24 * (double)acc.az.value());
         ei = c.e4 * (double)acc.ae.value();
-        if (e) stream_store(&e[i], (float)ei);
+        if (e) e[i] = (float)ei;
         vi = c.e24 * (double)acc.av.value();
     }
 }
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_batch_force(const float4* __restr
     double ei, vi;
     uint32_t np;
     batch_pairs<PRED, PREC, QUEUE>(pos, off, k, active, key, sys[s0], Rx, R, depth,
-                                   wave_lds_block<QUEUE>(queue, depth), fx,
+                                   queue + (threadIdx.x / kWaveSize) * depth * kWaveSize + __lane_id(), fx,
                                    fy, fz, e, ei, vi, np);
     const double es = wave_sum(ei), vs = wave_sum(vi);
     const unsigned long long ns = wave_sum((unsigned long long)np);
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kWaveSize) void k_batch_force_shared(
     double ei, vi;
     uint32_t np;
     batch_pairs<PRED, PREC, QUEUE>(pos, off, k, active, key, sys[find_sys(cbase, chint, nsys, key)], Rx, R, depth,
-                                   wave_lds_block<QUEUE>(queue, depth), fx, fy, fz, e,
+                                   queue + (threadIdx.x / kWaveSize) * depth * kWaveSize + __lane_id(), fx, fy, fz, e,
                                    ei, vi, np);
     if (active) {
         ev[k] = make_double2(ei, vi);
@@ -431,7 +431,7 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
         const unsigned nb = div_up(n, sh.block);
         const size_t lds = sh.lds_bytes();
-        const size_t lds_shared = sh.lds_per_wave();
+        const size_t lds_shared = sh.queued() ? (size_t)sh.depth * kWaveSize * sizeof(QEntry) : 0;
         if (n > 0) {
 #define LJ_BFQ(PRED, PREC, Q)                                                                                     \
     do {                                                                                                          \

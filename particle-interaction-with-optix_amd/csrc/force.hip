@@ -19,9 +19,9 @@ namespace lj {
 
 namespace {
 
-// One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues
-// (QUEUE: depth x block entries of 16 bytes, one block of rings per wave) or row tables
-// (kRowTable x block entries of 8 bytes) in dynamic LDS.  Outputs go to the caller's
+// One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues in
+// dynamic LDS (depth x block entries of 12 bytes, one block of rings per wave; none
+// without QUEUE).  Outputs go to the caller's
 // order (pi.w holds the original index); per-block FP64 energy and virial partials are
 // reduced later in a fixed order.
 template <int PRED, int PREC, bool VIR, bool QUEUE>
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict
                                                      int R, Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
                                                      float* __restrict__ fy, float* __restrict__ fz,
                                                      float* __restrict__ e, double* __restrict__ partials,
-                                                     unsigned long long* __restrict__ npairs) {
+                                                     unsigned long long* __restrict__ npairs, bool nt) {
     extern __shared__ QEntry queue[];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t k = blk * blockDim.x + threadIdx.x;
@@ -40,16 +40,18 @@ __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict
     acc.set_i(pi);
     acc.allow_exact_diff(c, pi, active);
     const uint32_t np = stencil_pairs<PRED, PREC, VIR, QUEUE>(pos, off, k, active, pi, g, 0u, active ? cell_of[k] : 0u,
-                                                              Rx, R, tr, c, wave_lds_block<QUEUE>(queue, depth),
+                                                              Rx, R, tr, c,
+                                                              queue + (threadIdx.x / kWaveSize) * depth * kWaveSize +
+                                                                  __lane_id(),
                                                               depth, acc);
     double esum = 0.0, vsum = 0.0;
     if (active) {
         const uint32_t i = __float_as_uint(pi.w);
-        stream_store(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
-        stream_store(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
-        stream_store(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
+        store_out(&fx[i], (float)(-c.e24 * (double)acc.ax.value()), nt);
+        store_out(&fy[i], (float)(-c.e24 * (double)acc.ay.value()), nt);
+        store_out(&fz[i], (float)(-c.e24 * (double)acc.az.value()), nt);
         const double ei = c.e4 * (double)acc.ae.value();
-        if (e) stream_store(&e[i], (float)ei);
+        if (e) store_out(&e[i], (float)ei, nt);
         esum = ei;
         vsum = c.e24 * (double)acc.av.value();
     }
@@ -179,7 +181,8 @@ void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t 
     const size_t lds = sh.lds_bytes();
 #define LJ_LAUNCH_Q(PREC, VIR, Q)                                                                                \
     hipLaunchKernelGGL((k_force<PRED, PREC, VIR, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
-                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs)
+                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs,    \
+                       non_temporal_outputs(n))
 #define LJ_LAUNCH(PREC, VIR)                   \
     do {                                       \
         if (sh.queued()) LJ_LAUNCH_Q(PREC, VIR, true); \

@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
                                                     float* __restrict__ fx, float* __restrict__ fy,
                                                     float* __restrict__ fz, float* __restrict__ e,
                                                     double* __restrict__ partials, lj_result* __restrict__ res,
-                                                    int* __restrict__ overflow) {
+                                                    int* __restrict__ overflow, bool nt) {
     extern __shared__ int lds_list[];
     int* list = lds_list + threadIdx.x;
     // consecutive queries walk nearly the same nodes: keep them on one XCD (own L2)
@@ -427,11 +427,11 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
         });
         if (!ok) atomicOr(overflow, 1);
         const uint32_t i = __float_as_uint(pi.w);
-        stream_store(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
-        stream_store(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
-        stream_store(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
+        store_out(&fx[i], (float)(-c.e24 * (double)acc.ax.value()), nt);
+        store_out(&fy[i], (float)(-c.e24 * (double)acc.ay.value()), nt);
+        store_out(&fz[i], (float)(-c.e24 * (double)acc.az.value()), nt);
         const double ei = c.e4 * (double)acc.ae.value();
-        if (e) stream_store(&e[i], (float)ei);
+        if (e) store_out(&e[i], (float)ei, nt);
         esum = ei;
         vsum = c.e24 * (double)acc.av.value();
     }
@@ -649,7 +649,7 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
 #define LJ_LB(PRED, PREC, VIR)                                                                                        \
     hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, fx, \
-                       fy, fz, e, partials, res, ovf)
+                       fy, fz, e, partials, res, ovf, non_temporal_outputs(n))
 #define LJ_LB_PREC(PRED)                                                                                    \
     switch (p->precision) {                                                                                 \
     case LJ_PREC_MIXED: if (vir) LJ_LB(PRED, LJ_PREC_MIXED, true); else LJ_LB(PRED, LJ_PREC_MIXED, false); break; \
@@ -666,7 +666,7 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
         }
     } else {
         hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true>), dim3(1), dim3(kLbvhBlock), lds, s, leaf, range, rope, box,
-                           0, h, lim, c, fx, fy, fz, e, partials, res, ovf);
+                           0, h, lim, c, fx, fy, fz, e, partials, res, ovf, false);
     }
 #undef LJ_LB_PREC
 #undef LJ_LB

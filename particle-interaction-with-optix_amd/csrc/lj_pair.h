@@ -139,10 +139,22 @@ struct PairSums {
 };
 
 // Store of one per-particle output at its caller index.  Sorted slots map to random
-// caller indices, so every such store is a partial write of its own line; a
-// non-temporal store does not keep that line in the caches, where it would displace the
-// neighbour positions the stencil loop re-reads (C3 force kernel 4.24 -> 4.09 ms).
-__device__ inline void stream_store(float* p, float v) { __builtin_nontemporal_store(v, p); }
+// caller indices, so every such store is a partial write of its own line.  When the
+// outputs fit in the L2s, ordinary stores let those lines fill up there before they go
+// to memory; when they do not, a non-temporal store keeps each line out of the caches,
+// where it would displace the neighbour positions the force loops re-read.  Measured
+// (force kernels, ordinary -> non-temporal): C2 1M particles 0.145 -> 0.161 ms, C5 4.2M
+// 0.91 -> 0.88 ms (LBVH 2.65 -> 2.60), C3 16.7M 4.26 -> 4.03 ms, and the batched C4,
+// whose systems each write a range of at most 256 KB, 8.1 -> 9.1 ms.  `nt` is
+// wave-uniform (non_temporal_outputs on the host).
+__device__ inline void store_out(float* p, float v, bool nt) {
+    if (nt) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// Whether a launch writing n particles' outputs (four floats each, at random indices)
+// should store them non-temporally: when they exceed the chip's L2 (8 XCDs x 4 MiB).
+inline bool non_temporal_outputs(int64_t n) { return (uint64_t)(n > 0 ? n : 0) * 16u > (32ull << 20); }
 
 template <class T>
 __device__ inline T wave_sum(T v) {

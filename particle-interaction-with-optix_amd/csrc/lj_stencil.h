@@ -37,9 +37,7 @@ struct Trim {
 struct Shape {
     int refine, refine_x, block, depth;
     bool queued() const { return depth > 1; }
-    // LDS of one wave: its lanes' rings (queued) or its lanes' row tables
-    size_t lds_per_wave() const;
-    size_t lds_bytes() const { return lds_per_wave() * (size_t)(block / 64); }
+    size_t lds_bytes() const { return queued() ? (size_t)depth * (size_t)block * 16 : 0; }
 };
 
 constexpr int kMaxBlock = 256;
@@ -103,46 +101,15 @@ struct __align__(16) QEntry {
     float x, y, z, pad;
 };
 
-// Row tables (QUEUE = false).  A wave runs each step of a loop for as long as its
-// longest lane needs, so a loop per row (or per mirror pair of rows) costs the sum over
-// rows of the longest lane's row.  Instead every lane lists its non-empty rows, in
-// visit order, as (first slot, end slot) in a table of kRowTable entries in LDS (entry
-// r of lane l at (r 64 + l) 8 bytes of its wave's block), and walks the table as one
-// stream of candidates, reading the next entry one row ahead; the table is walked and
-// emptied whenever a lane could not take another pair of rows, so a wave costs the sum
-// over these segments of its longest lane's total.  Modelled on C3 (16.7M uniform
-// particles, force grid rc/4 x rc/2 x rc/2, three candidates per step), steps per wave:
-// 93.7 by mirror pairs, 78.0 with 8-entry tables, 68.5 with 26 (one segment for the 25
-// rows of refine 2), 50.9 for perfectly balanced lanes.  The table's LDS bounds the
-// waves per CU, and that decides: measured C3 force kernel 4.05 ms by pairs, 4.00 ms
-// with 8 entries (8 waves per SIMD, the most the VGPRs allow), 4.45 ms with 14, 5.8 ms
-// with 26 (3 waves per SIMD: the loads' latency is no longer hidden).  The visit order,
-// hence the sums and their bits, are those of the loop by pairs and of QUEUE = true.
-constexpr int kRowTable = 8;
-
-// The LDS block of the calling thread's wave (dynamic LDS `lds`, blocks of
-// lds_per_wave bytes one after another).
-template <bool QUEUE>
-__device__ inline void* wave_lds_block(void* lds, uint32_t depth) {
-    const uint32_t per_wave = QUEUE ? depth * 64u * (uint32_t)sizeof(QEntry) : (uint32_t)(kRowTable * 64 * sizeof(uint2));
-    return static_cast<unsigned char*>(lds) + (threadIdx.x / kWaveSize) * per_wave;
-}
-
-inline size_t Shape::lds_per_wave() const {
-    return queued() ? (size_t)depth * 64 * sizeof(QEntry) : (size_t)kRowTable * 64 * sizeof(uint2);
-}
-
 template <int PRED, int PREC, bool VIR, bool QUEUE>
 __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const uint32_t* __restrict__ off, uint32_t k,
                                          bool active, const float4& pi, const GridGeom& g, uint32_t cbase,
                                          uint32_t key, int Rx, int R, const Trim& tr, const PairConst& c,
-                                         void* __restrict__ wave_lds, unsigned depth,
+                                         QEntry* __restrict__ queue, unsigned depth,
                                          PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR>& acc) {
     // only the TRIANGLE ray model counts a pair more than once (queued partners then
     // get their multiplicity again when drained)
     constexpr bool kMultiple = PRED == LJ_PRED_TRIANGLE;
-    QEntry* const queue = static_cast<QEntry*>(wave_lds) + __lane_id();  // this lane's ring (QUEUE)
-    uint2* const rows = static_cast<uint2*>(wave_lds) + __lane_id();      // this lane's row table
     unsigned pc = 0, head = 0;  // entries queued by this lane, index of the oldest
     uint32_t np = 0;
     auto push = [&](const float4& pj) {
@@ -238,65 +205,6 @@ __device__ inline uint32_t stencil_pairs(const float4* __restrict__ pos, const u
     // candidates.  A row and its mirror hold about as many candidates together wherever
     // the particle sits in its cell, so the lanes of a wave finish a pair at closer
     // times than a single row, and a wave runs each step for its longest lane.
-    if constexpr (!QUEUE) {
-        unsigned cnt = 0;  // entries in this lane's table
-        // walks the table: t..en is the current row, nt..ne the next one (read ahead)
-        auto walk = [&]() {
-            uint32_t t = 0, en = 0, nt = 0, ne = 0;
-            unsigned q = 0;
-            if (q < cnt) {
-                const uint2 r = rows[(q++) * kWaveSize];
-                t = r.x;
-                en = r.y;
-            }
-            if (q < cnt) {
-                const uint2 r = rows[(q++) * kWaveSize];
-                nt = r.x;
-                ne = r.y;
-            }
-            uint32_t enu[kCandidates];
-#pragma unroll
-            for (int u = 1; u < kCandidates; ++u) enu[u] = en > (uint32_t)u ? en - (uint32_t)u : 0u;
-            while (t < en) {
-                // slots up to t + kCandidates - 1 <= n + kCandidates - 2 exist (pos has
-                // spare entries); they count only below en
-                const float4* pt = pos + t;
-                float4 pj[kCandidates];
-#pragma unroll
-                for (int u = 0; u < kCandidates; ++u) pj[u] = pt[u];
-                visit(t, pj[0]);
-#pragma unroll
-                for (int u = 1; u < kCandidates; ++u)
-                    if (t < enu[u]) visit(t + u, pj[u]);
-                t += kCandidates;
-                if (t >= en) {  // on to the next row; t = en = 0 ends this lane's walk
-                    t = nt;
-                    en = ne;
-                    nt = ne = 0u;
-                    if (q < cnt) {
-                        const uint2 r = rows[(q++) * kWaveSize];
-                        nt = r.x;
-                        ne = r.y;
-                    }
-#pragma unroll
-                    for (int u = 1; u < kCandidates; ++u) enu[u] = en > (uint32_t)u ? en - (uint32_t)u : 0u;
-                }
-            }
-            cnt = 0;
-        };
-        for (int32_t dz = 0; dz <= R; ++dz)
-            for (int32_t dy = dz == 0 ? 0 : -R; dy <= R; ++dy) {
-                uint32_t b, en, b2, e2;
-                row_range(dz, dy, b, en);
-                if (dz == 0 && dy == 0) b2 = e2 = 0u;
-                else row_range(-dz, -dy, b2, e2);
-                if (b < en) rows[(cnt++) * kWaveSize] = make_uint2(b, en);
-                if (b2 < e2) rows[(cnt++) * kWaveSize] = make_uint2(b2, e2);
-                if (__any(cnt + 2 > (unsigned)kRowTable)) walk();  // every lane takes part here
-            }
-        walk();
-        return np;
-    }
     for (int32_t dz = 0; dz <= R; ++dz)
         for (int32_t dy = dz == 0 ? 0 : -R; dy <= R; ++dy) {
             if constexpr (QUEUE) {  // every lane takes part here
