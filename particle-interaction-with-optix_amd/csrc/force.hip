@@ -24,13 +24,13 @@ namespace {
 // without QUEUE).  Outputs go to the caller's
 // order (pi.w holds the original index); per-block FP64 energy and virial partials are
 // reduced later in a fixed order.
-template <int PRED, int PREC, bool VIR, bool QUEUE>
+template <int PRED, int PREC, bool VIR, bool QUEUE, bool NT>
 __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
                                                      const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int Rx,
                                                      int R, Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
                                                      float* __restrict__ fy, float* __restrict__ fz,
                                                      float* __restrict__ e, double* __restrict__ partials,
-                                                     unsigned long long* __restrict__ npairs, bool nt) {
+                                                     unsigned long long* __restrict__ npairs) {
     extern __shared__ QEntry queue[];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t k = blk * blockDim.x + threadIdx.x;
@@ -47,11 +47,11 @@ __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict
     double esum = 0.0, vsum = 0.0;
     if (active) {
         const uint32_t i = __float_as_uint(pi.w);
-        store_out(&fx[i], (float)(-c.e24 * (double)acc.ax.value()), nt);
-        store_out(&fy[i], (float)(-c.e24 * (double)acc.ay.value()), nt);
-        store_out(&fz[i], (float)(-c.e24 * (double)acc.az.value()), nt);
+        store_out<NT>(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
+        store_out<NT>(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
+        store_out<NT>(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
         const double ei = c.e4 * (double)acc.ae.value();
-        if (e) store_out(&e[i], (float)ei, nt);
+        if (e) store_out<NT>(&e[i], (float)ei);
         esum = ei;
         vsum = c.e24 * (double)acc.av.value();
     }
@@ -179,10 +179,15 @@ void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t 
                   const PairConst& c, float* fx, float* fy, float* fz, float* e, double* partials,
                   unsigned long long* npairs) {
     const size_t lds = sh.lds_bytes();
-#define LJ_LAUNCH_Q(PREC, VIR, Q)                                                                                \
-    hipLaunchKernelGGL((k_force<PRED, PREC, VIR, Q>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
-                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs,    \
-                       non_temporal_outputs(n))
+    const bool nt = non_temporal_outputs(n);
+#define LJ_LAUNCH_QN(PREC, VIR, Q, NT)                                                                              \
+    hipLaunchKernelGGL((k_force<PRED, PREC, VIR, Q, NT>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
+                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs)
+#define LJ_LAUNCH_Q(PREC, VIR, Q)                 \
+    do {                                          \
+        if (nt) LJ_LAUNCH_QN(PREC, VIR, Q, true); \
+        else LJ_LAUNCH_QN(PREC, VIR, Q, false);   \
+    } while (0)
 #define LJ_LAUNCH(PREC, VIR)                   \
     do {                                       \
         if (sh.queued()) LJ_LAUNCH_Q(PREC, VIR, true); \
@@ -195,6 +200,7 @@ void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t 
     }
 #undef LJ_LAUNCH
 #undef LJ_LAUNCH_Q
+#undef LJ_LAUNCH_QN
 }
 
 }  // namespace

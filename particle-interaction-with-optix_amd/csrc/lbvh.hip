@@ -398,13 +398,13 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
     return ok;
 }
 
-template <int PRED, int PREC, bool VIR>
+template <int PRED, int PREC, bool VIR, bool NT>
 __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                                     const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, uint32_t limit, PairConst c,
                                                     float* __restrict__ fx, float* __restrict__ fy,
                                                     float* __restrict__ fz, float* __restrict__ e,
                                                     double* __restrict__ partials, lj_result* __restrict__ res,
-                                                    int* __restrict__ overflow, bool nt) {
+                                                    int* __restrict__ overflow) {
     extern __shared__ int lds_list[];
     int* list = lds_list + threadIdx.x;
     // consecutive queries walk nearly the same nodes: keep them on one XCD (own L2)
@@ -427,11 +427,11 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
         });
         if (!ok) atomicOr(overflow, 1);
         const uint32_t i = __float_as_uint(pi.w);
-        store_out(&fx[i], (float)(-c.e24 * (double)acc.ax.value()), nt);
-        store_out(&fy[i], (float)(-c.e24 * (double)acc.ay.value()), nt);
-        store_out(&fz[i], (float)(-c.e24 * (double)acc.az.value()), nt);
+        store_out<NT>(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
+        store_out<NT>(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
+        store_out<NT>(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
         const double ei = c.e4 * (double)acc.ae.value();
-        if (e) store_out(&e[i], (float)ei, nt);
+        if (e) store_out<NT>(&e[i], (float)ei);
         esum = ei;
         vsum = c.e24 * (double)acc.av.value();
     }
@@ -647,9 +647,15 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     const int* rope = ctx->lbvh.rope.as<int>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
-#define LJ_LB(PRED, PREC, VIR)                                                                                        \
-    hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR>), dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, (int)n, h, lim, c, fx, \
-                       fy, fz, e, partials, res, ovf, non_temporal_outputs(n))
+    const bool nt = non_temporal_outputs(n);
+#define LJ_LBN(PRED, PREC, VIR, NT)                                                                                  \
+    hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR, NT>), dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, \
+                       (int)n, h, lim, c, fx, fy, fz, e, partials, res, ovf)
+#define LJ_LB(PRED, PREC, VIR)                  \
+    do {                                        \
+        if (nt) LJ_LBN(PRED, PREC, VIR, true);  \
+        else LJ_LBN(PRED, PREC, VIR, false);    \
+    } while (0)
 #define LJ_LB_PREC(PRED)                                                                                    \
     switch (p->precision) {                                                                                 \
     case LJ_PREC_MIXED: if (vir) LJ_LB(PRED, LJ_PREC_MIXED, true); else LJ_LB(PRED, LJ_PREC_MIXED, false); break; \
@@ -665,11 +671,12 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
         case LJ_PRED_TRIANGLE: LJ_LB_PREC(LJ_PRED_TRIANGLE); break;
         }
     } else {
-        hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true>), dim3(1), dim3(kLbvhBlock), lds, s, leaf, range, rope, box,
-                           0, h, lim, c, fx, fy, fz, e, partials, res, ovf, false);
+        hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true, false>), dim3(1), dim3(kLbvhBlock), lds, s, leaf,
+                           range, rope, box, 0, h, lim, c, fx, fy, fz, e, partials, res, ovf);
     }
 #undef LJ_LB_PREC
 #undef LJ_LB
+#undef LJ_LBN
     finalize_energy(partials, nb, res, s);
     hipLaunchKernelGGL(k_poison_on_overflow, dim3(1), dim3(64), 0, s, (const int*)ovf, res);
     LJ_HIP(ctx, hipGetLastError());

@@ -145,10 +145,13 @@ struct PairSums {
 // where it would displace the neighbour positions the force loops re-read.  Measured
 // (force kernels, ordinary -> non-temporal): C2 1M particles 0.145 -> 0.161 ms, C5 4.2M
 // 0.91 -> 0.88 ms (LBVH 2.65 -> 2.60), C3 16.7M 4.26 -> 4.03 ms, and the batched C4,
-// whose systems each write a range of at most 256 KB, 8.1 -> 9.1 ms.  `nt` is
-// wave-uniform (non_temporal_outputs on the host).
-__device__ inline void store_out(float* p, float v, bool nt) {
-    if (nt) __builtin_nontemporal_store(v, p);
+// whose systems each write a range of at most 256 KB, 8.1 -> 9.1 ms.  The kind is a
+// template parameter of the kernels (non_temporal_outputs on the host picks the
+// instance): with a run-time branch between the two forms the compiler merges them into
+// one ordinary store.
+template <bool NT>
+__device__ inline void store_out(float* p, float v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
     else *p = v;
 }
 
