@@ -16,6 +16,11 @@ ordered pairs / step time.  The JSON line also carries:
                 one-thread figure on a sample), with the CPU model
   batched       the 4,096-system batch: one call over all systems (eager and
                 replayed from a captured graph) against one call per system
+  sorted_mode   the same step through lj_compute_sorted, which leaves the particles in
+                cell order (the reference's data flow, cudaLJ.cu:441-471) so every
+                output store is consecutive: with the shuffled input copied back before
+                each step inside the timed region, and in the steady state of a
+                simulation whose particles stay in the order the last step left them
 Before the timed steps the kernel shape (force-grid refinement, particles per
 workgroup, LDS queue depth) is picked by the engine's autotuner on the same input,
 outside the timed region; the chosen shape is reported under config.shape.
@@ -53,6 +58,42 @@ def time_steps(torch, fn, steps):
     stop.record()
     torch.cuda.synchronize()
     return start.elapsed_time(stop) / steps
+
+
+def sorted_mode(torch, ljamd, eng, dx, dy, dz, p, pairs, steps, warmup):
+    """lj_compute_sorted on the workload of the headline step (same shape).  'restored':
+    the shuffled positions are copied back before every step, inside the timed region,
+    so each step sorts the same input as the headline; 'steady': the positions stay in
+    the cell order the previous step left them."""
+    sx, sy, sz = dx.clone(), dy.clone(), dz.clone()
+    out = eng.compute_sorted(sx, sy, sz, p)
+
+    def restored():
+        sx.copy_(dx)
+        sy.copy_(dy)
+        sz.copy_(dz)
+        eng.compute_sorted(sx, sy, sz, p, out=out)
+
+    def steady():
+        eng.compute_sorted(sx, sy, sz, p, out=out)
+
+    res = {}
+    for name, fn in (("restored", restored), ("steady", steady)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        eng.profile(True)
+        ms = time_steps(torch, fn, steps)
+        ph = eng.profile_read()
+        eng.profile(False)
+        r = ljamd.read_results(out["result"])[0]
+        if int(r["n_pairs"]) != pairs:
+            raise RuntimeError(f"sorted mode found {int(r['n_pairs'])} pairs, the headline step {pairs}")
+        res[name] = {"ms_per_step": ms, "value": pairs / (ms / 1e3),
+                     "ms_force": ph["force"][0] / max(ph["force"][1], 1),
+                     "ms_bin": ph["bin"][0] / max(ph["bin"][1], 1)}
+    del out, sx, sy, sz
+    return res
 
 
 def traffic_of(kernel_prefix: str, workload: str, p):
@@ -359,6 +400,8 @@ def main():
         t0 = time.perf_counter()
         line["accuracy"], line["cpu_baseline"] = oracle_leg(ljamd, args.workload, w, out, args.cpu_threads)
         log(f"cpu baseline {time.perf_counter() - t0:.1f} s")
+    if not args.lbvh:
+        line["sorted_mode"] = sorted_mode(torch, ljamd, eng, dx, dy, dz, w.params, pairs, args.steps, args.warmup)
     del out
     eng.close()
     if not args.no_batched:

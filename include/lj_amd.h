@@ -180,6 +180,21 @@ int lj_compute(lj_context* ctx, const float* x, const float* y, const float* z, 
                const lj_params* p, float* fx, float* fy, float* fz, float* e,
                lj_result* result, void* stream);
 
+/* Device: lj_compute with the particles left in cell order, the data flow of the
+ * reference's loop, which moves its particles into cell order every step
+ * (MoveParticlesToNewCells, cudaLJ.cu:441-471; InitNewCells, :428-439) before the
+ * energy kernel writes per-particle values in that order (cudaLJ.cu:299-341, 649-653).
+ * On return x,y,z[n] hold the positions in the stable cell order of lj_bin (read first,
+ * then overwritten), fx,fy,fz[n] and e[n] (nullable) the outputs of those particles in
+ * the same order, and perm[n] (nullable) the index each particle had on input.  The
+ * outputs are bit for bit those of lj_compute on the same input, permuted by perm; all
+ * writes are consecutive, where lj_compute's go to the shuffled input order.  A caller
+ * that keeps its particle state in this order also feeds the next step's sort an
+ * almost sorted input.  Outputs must not alias the positions. */
+int lj_compute_sorted(lj_context* ctx, float* x, float* y, float* z, int64_t n, const lj_params* p,
+                      float* fx, float* fy, float* fz, float* e, uint32_t* perm, lj_result* result,
+                      void* stream);
+
 /* Device: lj_compute without any device allocation.  No counterpart in the reference:
  * for callers that cannot allocate during a call (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels and results

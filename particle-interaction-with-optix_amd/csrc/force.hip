@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict
                                                      int R, Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
                                                      float* __restrict__ fy, float* __restrict__ fz,
                                                      float* __restrict__ e, double* __restrict__ partials,
-                                                     unsigned long long* __restrict__ npairs) {
+                                                     unsigned long long* __restrict__ npairs, SortedOut so) {
     extern __shared__ QEntry queue[];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t k = blk * blockDim.x + threadIdx.x;
@@ -46,7 +46,15 @@ __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict
                                                               depth, acc);
     double esum = 0.0, vsum = 0.0;
     if (active) {
-        const uint32_t i = __float_as_uint(pi.w);
+        // caller order, or the sorted slot together with the particle itself
+        const bool sorted = so.x != nullptr;  // uniform
+        const uint32_t i = sorted ? k : __float_as_uint(pi.w);
+        if (sorted) {
+            store_out<NT>(&so.x[k], pi.x);
+            store_out<NT>(&so.y[k], pi.y);
+            store_out<NT>(&so.z[k], pi.z);
+            if (so.perm) so.perm[k] = __float_as_uint(pi.w);
+        }
         store_out<NT>(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
         store_out<NT>(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
         store_out<NT>(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
@@ -177,12 +185,12 @@ template <int PRED>
 void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t s, const float4* pos,
                   const uint32_t* cell_of, const uint32_t* off, uint32_t n, const GridGeom& g, const Trim& tr,
                   const PairConst& c, float* fx, float* fy, float* fz, float* e, double* partials,
-                  unsigned long long* npairs) {
+                  unsigned long long* npairs, const SortedOut& so) {
     const size_t lds = sh.lds_bytes();
     const bool nt = non_temporal_outputs(n);
 #define LJ_LAUNCH_QN(PREC, VIR, Q, NT)                                                                              \
     hipLaunchKernelGGL((k_force<PRED, PREC, VIR, Q, NT>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
-                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs)
+                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs, so)
 #define LJ_LAUNCH_Q(PREC, VIR, Q)                 \
     do {                                          \
         if (nt) LJ_LAUNCH_QN(PREC, VIR, Q, true); \
@@ -298,7 +306,7 @@ Trim make_trim(const lj_params* p, const GridGeom& g) {
 
 int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const lj_params* p,
                      const GridGeom& g, const PairConst& c, float* fx, float* fy, float* fz, float* e,
-                     lj_result* result, hipStream_t s) {
+                     lj_result* result, hipStream_t s, const SortedOut& so) {
     const Shape sh = choose_shape(p, n, g.ncells);
     const bool refined = sh.refine > 1 || sh.refine_x > 1;
     GridGeom fg = g;
@@ -342,7 +350,7 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
 #define LJ_PRED_CASE(P)                                                                                         \
     case P:                                                                                                     \
         launch_force<P>(prec, vir, sh, nb, s, pos, cell_of, off, nn, fg, tr, c, fx, fy, fz, e, partials,      \
-                        &res->n_pairs);                                                                         \
+                        &res->n_pairs, so);                                                                     \
         break;
         switch (p->predicate) {
             LJ_PRED_CASE(LJ_PRED_AABB)
@@ -417,6 +425,24 @@ int lj_compute(lj_context* ctx, const float* x, const float* y, const float* z, 
         return fail(ctx, LJ_ERR_INVALID_ARG, "positions and forces must not be NULL");
     LJ_HIP(ctx, hipSetDevice(ctx->device));
     return compute_cellgrid(ctx, x, y, z, n, p, g, c, fx, fy, fz, e, result, (hipStream_t)stream);
+}
+
+int lj_compute_sorted(lj_context* ctx, float* x, float* y, float* z, int64_t n, const lj_params* p, float* fx,
+                      float* fy, float* fz, float* e, uint32_t* perm, lj_result* result, void* stream) {
+    if (!ctx) return LJ_ERR_INVALID_ARG;
+    GridGeom g;
+    PairConst c;
+    LJ_TRY(make_geom(ctx, p, n, &g, &c));
+    if (n > 0 && (!x || !y || !z || !fx || !fy || !fz))
+        return fail(ctx, LJ_ERR_INVALID_ARG, "positions and forces must not be NULL");
+    LJ_HIP(ctx, hipSetDevice(ctx->device));
+    // the sort reads x, y, z before the force launch overwrites them
+    SortedOut so;
+    so.x = x;
+    so.y = y;
+    so.z = z;
+    so.perm = perm;
+    return compute_cellgrid(ctx, x, y, z, n, p, g, c, fx, fy, fz, e, result, (hipStream_t)stream, so);
 }
 
 int lj_pairs(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const lj_params* p,

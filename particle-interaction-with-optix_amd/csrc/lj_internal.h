@@ -188,9 +188,18 @@ int scan_u32(lj_context* ctx, const uint32_t* in, uint32_t* out, int64_t n, uint
 int scan_u32_to_u64(lj_context* ctx, const uint32_t* in, unsigned long long* out, int64_t n, hipStream_t s);
 
 // Cell-grid pipelines (force.hip).
+// Where a force launch also leaves the particles in cell order (lj_compute_sorted):
+// positions and original indices by sorted slot, outputs by sorted slot too.  All null:
+// outputs by original index.
+struct SortedOut {
+    float* x = nullptr;
+    float* y = nullptr;
+    float* z = nullptr;
+    uint32_t* perm = nullptr;
+};
 int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const float* z, int64_t n, const lj_params* p,
                      const GridGeom& g, const PairConst& c, float* fx, float* fy, float* fz, float* e,
-                     lj_result* result, hipStream_t s);
+                     lj_result* result, hipStream_t s, const SortedOut& so = SortedOut{});
 
 // Key functor of the cell grid (for sort_particles in lj_sort.h).
 struct CellKeyOp {

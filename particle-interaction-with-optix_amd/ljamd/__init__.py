@@ -64,6 +64,7 @@ EXPORTS = (
     "lj_max_displacement",
     "lj_workspace_generation",
     "lj_compute_noalloc",
+    "lj_compute_sorted",
 )
 PHASES = ("bin", "force", "other")
 
@@ -170,6 +171,7 @@ def load(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
         "lj_max_displacement": ([vp, vp, vp, vp, vp, vp, vp, i64, vp, vp], ctypes.c_int),
         "lj_workspace_generation": ([vp, P(u64)], ctypes.c_int),
         "lj_compute_noalloc": ([vp, vp, vp, vp, i64, P(LJParams), vp, vp, vp, vp, vp, vp], ctypes.c_int),
+        "lj_compute_sorted": ([vp, vp, vp, vp, i64, P(LJParams), vp, vp, vp, vp, vp, vp, vp], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         if hasattr(lib, name):  # tests check that every declared symbol is exported
@@ -360,6 +362,27 @@ class Engine:
         self._check(
             fn(self.ctx, _ptr(x), _ptr(y), _ptr(z), n, ctypes.byref(p), _ptr(out["fx"]), _ptr(out["fy"]), _ptr(out["fz"]),
                _ptr(out["e"]), _ptr(out["result"]), self._stream())
+        )
+        return out
+
+    def compute_sorted(self, x, y, z, p: LJParams, energies: bool = True, out=None):
+        """lj_compute_sorted: forces and energies with the particles left in cell order.
+        x, y, z are overwritten with the sorted positions; the returned fx, fy, fz, e
+        are in that order and 'perm' holds each particle's input index."""
+        n = x.numel()
+        if out is None:
+            out = {
+                "fx": self.empty_f32(n),
+                "fy": self.empty_f32(n),
+                "fz": self.empty_f32(n),
+                "e": self.empty_f32(n) if energies else None,
+                "perm": self.torch.empty(max(n, 1), dtype=self.torch.int32, device=x.device),
+                "result": self.empty_result(),
+            }
+        self._check(
+            self.lib.lj_compute_sorted(self.ctx, _ptr(x), _ptr(y), _ptr(z), n, ctypes.byref(p), _ptr(out["fx"]),
+                                       _ptr(out["fy"]), _ptr(out["fz"]), _ptr(out["e"]), _ptr(out.get("perm")),
+                                       _ptr(out["result"]), self._stream())
         )
         return out
 
