@@ -20,9 +20,10 @@ namespace lj {
 namespace {
 
 // One thread per sorted slot k, blocks of `block` threads (<= 256), the lane queues in
-// dynamic LDS (depth x block entries of 12 bytes, one block of rings per wave; none
-// without QUEUE).  Outputs go to the caller's
-// order (pi.w holds the original index); per-block FP64 energy and virial partials are
+// dynamic LDS (depth x block entries of 16 bytes, one block of rings per wave; none
+// without QUEUE).  Outputs go to the caller's order (pi.w holds the original index),
+// or, when `so` is set (lj_compute_sorted), to the sorted slot k together with the
+// particle's position and original index; per-block FP64 energy and virial partials are
 // reduced later in a fixed order.
 template <int PRED, int PREC, bool VIR, bool QUEUE, bool NT>
 __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,

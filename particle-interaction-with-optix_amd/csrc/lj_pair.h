@@ -138,8 +138,9 @@ struct PairSums {
     }
 };
 
-// Store of one per-particle output at its caller index.  Sorted slots map to random
-// caller indices, so every such store is a partial write of its own line.  When the
+// Store of one per-particle output at its caller index (or, for lj_compute_sorted, at
+// its sorted slot, where the stores are consecutive).  Sorted slots map to random
+// caller indices, so every caller-order store is a partial write of its own line.  When the
 // outputs fit in the L2s, ordinary stores let those lines fill up there before they go
 // to memory; when they do not, a non-temporal store keeps each line out of the caches,
 // where it would displace the neighbour positions the force loops re-read.  Measured
