@@ -305,8 +305,8 @@ __global__ __launch_bounds__(kNodeBoundsBlock) void k_node_bounds(const float4* 
 // the test never rejects a point within h.  For finite values this is exactly the six
 // one-sided tests.  fmaxf drops a NaN difference when the other operand is a number, so
 // the test then accepts more nodes, never fewer, and the leaf test decides; only when
-// every difference of an axis is NaN (a NaN query, or bounds no finite leaf made) does
-// it reject, and then no leaf passes the leaf test either.  The max form is 6 subtractions,
+// all six differences are NaN (a NaN query, or bounds no finite leaf made) does it
+// reject, and then no leaf passes the leaf test either.  The max form is 6 subtractions,
 // 2 three-way maxima and one compare, where six compares joined by && become a nest of
 // divergent branches.
 __device__ inline bool overlaps(float4 p, float h, float4 lo, float4 hi) {
