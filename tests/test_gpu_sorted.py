@@ -69,3 +69,28 @@ def test_compute_sorted_equals_compute_permuted(engine, name):
     assert np.array_equal(out2["perm"][:n].cpu().numpy(), np.arange(n, dtype=np.int32))
     for k in ("fx", "fy", "fz", "e"):
         assert torch.equal(out2[k][:n].view(torch.int32), out[k][:n].view(torch.int32)), k
+
+
+@pytest.mark.parametrize("pred,depth", [("cudalj", 1), ("triangle", 8), ("custom", 16)])
+def test_compute_sorted_other_predicates_and_queues(engine, pred, depth):
+    """The CUDALJ mode (every candidate, itself skipped by slot), a ray model and the LDS
+    queue of accepted partners give the same bits in sorted order as lj_compute's,
+    permuted."""
+    x, y, z = ljamd.generate("uniform", 91, 50_000)
+    p = ljamd.make_params(0.05, predicate=pred, depth=depth, block=128)
+    if pred == "cudalj":
+        p.softening2, p.sigma = 1e-5, 1.0
+    n = x.size
+    print(f"[sorted] {pred} depth={depth} n={n}", flush=True)
+    torch = engine.torch
+    dx, dy, dz = engine.to_device(x, y, z)
+    ref = engine.compute(dx, dy, dz, p)
+    sx, sy, sz = (t.clone() for t in (dx, dy, dz))
+    out = engine.compute_sorted(sx, sy, sz, p)
+    torch.cuda.synchronize()
+    assert ljamd.read_results(ref["result"])[0].tobytes() == ljamd.read_results(out["result"])[0].tobytes()
+    perm = out["perm"][:n].cpu().numpy().view(np.uint32)
+    for k in ("fx", "fy", "fz", "e"):
+        a = out[k][:n].cpu().numpy().view(np.uint32)
+        b = ref[k][:n].cpu().numpy()[perm].view(np.uint32)
+        assert np.array_equal(a, b), k
