@@ -109,6 +109,28 @@ def test_lbvh_walk_guard_marks_results(engine):
     assert int(res["n_pairs"]) == ref["n_pairs"] and np.isfinite(res["e_total"])
 
 
+@pytest.mark.parametrize("name", ["clusters_256k", "surface_100k"])
+def test_lbvh_deterministic_bits(engine, name):
+    """Two LBVH calls on the same input give the same bits: forces, energies, the
+    result record and the pair lists (the walk order is fixed by the tree, the totals
+    are reduced in a fixed order, the counts are integers)."""
+    (x, y, z), p = _lbvh_case(name)
+    print(f"[lbvh determinism] {name} n={x.size}", flush=True)
+    dx, dy, dz = engine.to_device(x, y, z)
+    got = []
+    for _ in range(2):
+        out = engine.compute(dx, dy, dz, p, lbvh=True)
+        rows, cols = engine.pairs(dx, dy, dz, p, lbvh=True)
+        engine.torch.cuda.synchronize()
+        got.append(([out[k][: x.size].cpu().numpy().view(np.uint32) for k in ("fx", "fy", "fz", "e")],
+                    out["result"].cpu().numpy().tobytes(), rows.cpu().numpy(), cols.cpu().numpy()))
+    (f0, r0, rows0, cols0), (f1, r1, rows1, cols1) = got
+    for a, b in zip(f0, f1):
+        assert np.array_equal(a, b)
+    assert r0 == r1
+    assert np.array_equal(rows0, rows1) and np.array_equal(cols0, cols1)
+
+
 def _small_batch(nsys=48):
     return workloads.batch_c4(nsys=nsys, nmin=64, nmax=4096, seed=77)
 
