@@ -19,7 +19,7 @@ template <class F>
 void for_each_buffer(lj_context* ctx, F&& f) {
     BinWork& b = ctx->bin;
     for (DevBuf* d : {&b.keys[0], &b.keys[1], &b.hist, &b.scan_tmp, &b.gaps, &b.offsets, &b.pos, &b.pos_alt,
-                      &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets, &b.cand})
+                      &b.partials, &b.counters, &b.rows, &b.ccounts, &b.coffsets, &b.cand, &b.stage})
         f(*d);
     LbvhWork& l = ctx->lbvh;
     for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags, &l.bucket_root}) f(*d);

@@ -25,16 +25,23 @@ namespace {
 // or, when `so` is set (lj_compute_sorted), to the sorted slot k together with the
 // particle's position and original index; per-block FP64 energy and virial partials are
 // reduced later in a fixed order.
-template <int PRED, int PREC, bool VIR, bool QUEUE, bool NT>
+// OUT: how the outputs are stored -- 0 ordinary stores, 1 non-temporal stores, 2 one
+// non-temporal staged record per particle at its caller index (large outputs at
+// shuffled indices; never with `so`).
+template <int PRED, int PREC, bool VIR, bool QUEUE, int OUT>
 __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict__ pos, const uint32_t* __restrict__ cell_of,
                                                      const uint32_t* __restrict__ off, uint32_t n, GridGeom g, int Rx,
                                                      int R, Trim tr, PairConst c, uint32_t depth, float* __restrict__ fx,
                                                      float* __restrict__ fy, float* __restrict__ fz,
                                                      float* __restrict__ e, double* __restrict__ partials,
-                                                     unsigned long long* __restrict__ npairs, SortedOut so) {
+                                                     unsigned long long* __restrict__ npairs, SortedOut so,
+                                                     float4* __restrict__ stage) {
     extern __shared__ QEntry queue[];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t k = blk * blockDim.x + threadIdx.x;
+    // the slot rebuilt after the pair loop from wave-uniform values and the lane id, so
+    // that no per-lane index stays live across it (64 VGPRs are the budget)
+    const uint32_t wbase = blk * blockDim.x + (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~(kWaveSize - 1u));
     const bool active = k < n;
     PairSums<PREC, PRED == LJ_PRED_CUDALJ, VIR> acc;
     const float4 pi = active ? pos[k] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -46,21 +53,31 @@ __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict
                                                                   __lane_id(),
                                                               depth, acc);
     double esum = 0.0, vsum = 0.0;
-    if (active) {
+    const uint32_t ks = wbase + __lane_id();  // = k
+    if (ks < n) {
         // caller order, or the sorted slot together with the particle itself
-        const bool sorted = so.x != nullptr;  // uniform
-        const uint32_t i = sorted ? k : __float_as_uint(pi.w);
-        if (sorted) {
-            store_out<NT>(&so.x[k], pi.x);
-            store_out<NT>(&so.y[k], pi.y);
-            store_out<NT>(&so.z[k], pi.z);
-            if (so.perm) so.perm[k] = __float_as_uint(pi.w);
-        }
-        store_out<NT>(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
-        store_out<NT>(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
-        store_out<NT>(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
+        const float4 ps = pos[ks];
+        const float ox = (float)(-c.e24 * (double)acc.ax.value());
+        const float oy = (float)(-c.e24 * (double)acc.ay.value());
+        const float oz = (float)(-c.e24 * (double)acc.az.value());
         const double ei = c.e4 * (double)acc.ae.value();
-        if (e) store_out<NT>(&e[i], (float)ei);
+        if constexpr (OUT == 2) {
+            store_staged(&stage[__float_as_uint(ps.w)], ox, oy, oz, (float)ei);
+        } else {
+            constexpr bool NT = OUT == 1;
+            const bool sorted = so.x != nullptr;  // uniform
+            const uint32_t i = sorted ? ks : __float_as_uint(ps.w);
+            if (sorted) {
+                store_out<NT>(&so.x[ks], ps.x);
+                store_out<NT>(&so.y[ks], ps.y);
+                store_out<NT>(&so.z[ks], ps.z);
+                if (so.perm) so.perm[ks] = __float_as_uint(ps.w);
+            }
+            store_out<NT>(&fx[i], ox);
+            store_out<NT>(&fy[i], oy);
+            store_out<NT>(&fz[i], oz);
+            if (e) store_out<NT>(&e[i], (float)ei);
+        }
         esum = ei;
         vsum = c.e24 * (double)acc.av.value();
     }
@@ -71,6 +88,18 @@ __global__ __launch_bounds__(kMaxBlock, 8) void k_force(const float4* __restrict
         partials[2 * blk + 1] = vsum;
         if (npairs && bp) atomicAdd(npairs, bp);
     }
+}
+
+__global__ __launch_bounds__(256) void k_split_outputs(const float4* __restrict__ stage, uint32_t n,
+                                                       float* __restrict__ fx, float* __restrict__ fy,
+                                                       float* __restrict__ fz, float* __restrict__ e) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(stage) + i);
+    fx[i] = v.x;
+    fy[i] = v.y;
+    fz[i] = v.z;
+    if (e) e[i] = v.w;
 }
 
 __global__ __launch_bounds__(256) void k_coarse_counts(const uint32_t* __restrict__ foff, GridGeom fg, GridGeom cg, int Kx,
@@ -186,16 +215,17 @@ template <int PRED>
 void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t s, const float4* pos,
                   const uint32_t* cell_of, const uint32_t* off, uint32_t n, const GridGeom& g, const Trim& tr,
                   const PairConst& c, float* fx, float* fy, float* fz, float* e, double* partials,
-                  unsigned long long* npairs, const SortedOut& so) {
+                  unsigned long long* npairs, const SortedOut& so, float4* stage) {
     const size_t lds = sh.lds_bytes();
-    const bool nt = non_temporal_outputs(n);
-#define LJ_LAUNCH_QN(PREC, VIR, Q, NT)                                                                              \
-    hipLaunchKernelGGL((k_force<PRED, PREC, VIR, Q, NT>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
-                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs, so)
-#define LJ_LAUNCH_Q(PREC, VIR, Q)                 \
-    do {                                          \
-        if (nt) LJ_LAUNCH_QN(PREC, VIR, Q, true); \
-        else LJ_LAUNCH_QN(PREC, VIR, Q, false);   \
+    const int out = stage ? 2 : non_temporal_outputs(n) ? 1 : 0;
+#define LJ_LAUNCH_QN(PREC, VIR, Q, OUT)                                                                              \
+    hipLaunchKernelGGL((k_force<PRED, PREC, VIR, Q, OUT>), dim3(nb), dim3(sh.block), lds, s, pos, cell_of, off, n, g, \
+                       sh.refine_x, sh.refine, tr, c, (uint32_t)sh.depth, fx, fy, fz, e, partials, npairs, so, stage)
+#define LJ_LAUNCH_Q(PREC, VIR, Q)                      \
+    do {                                               \
+        if (out == 2) LJ_LAUNCH_QN(PREC, VIR, Q, 2);   \
+        else if (out == 1) LJ_LAUNCH_QN(PREC, VIR, Q, 1); \
+        else LJ_LAUNCH_QN(PREC, VIR, Q, 0);            \
     } while (0)
 #define LJ_LAUNCH(PREC, VIR)                   \
     do {                                       \
@@ -213,6 +243,11 @@ void launch_force(int prec, bool vir, const Shape& sh, unsigned nb, hipStream_t 
 }
 
 }  // namespace
+
+void split_outputs(const float4* stage, uint32_t n, float* fx, float* fy, float* fz, float* e, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_split_outputs, dim3(div_up(n, 256)), dim3(256), 0, s, stage, n, fx, fy, fz, e);
+}
 
 void finalize_energy(double* partials, unsigned nblocks, lj_result* res, hipStream_t s) {
     if (nblocks > 1024) {
@@ -346,12 +381,18 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
     const uint32_t* cell_of = ctx->bin.sorted_keys();
     const uint32_t* off = ctx->bin.offsets.as<uint32_t>();
     const uint32_t nn = (uint32_t)n;
+    // large outputs at shuffled indices go through the staging array
+    float4* stage = nullptr;
+    if (non_temporal_outputs(n) && !so.x) {
+        LJ_HIP(ctx, ctx->bin.stage.grow((size_t)n * sizeof(float4)));
+        stage = ctx->bin.stage.as<float4>();
+    }
     {
         PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
 #define LJ_PRED_CASE(P)                                                                                         \
     case P:                                                                                                     \
         launch_force<P>(prec, vir, sh, nb, s, pos, cell_of, off, nn, fg, tr, c, fx, fy, fz, e, partials,      \
-                        &res->n_pairs, so);                                                                     \
+                        &res->n_pairs, so, stage);                                                              \
         break;
         switch (p->predicate) {
             LJ_PRED_CASE(LJ_PRED_AABB)
@@ -363,6 +404,7 @@ int compute_cellgrid(lj_context* ctx, const float* x, const float* y, const floa
 #undef LJ_PRED_CASE
     }
     PhaseTimer to(ctx, LJ_PHASE_OTHER, s);
+    if (stage) split_outputs(stage, nn, fx, fy, fz, e, s);
     finalize_energy(partials, nb, res, s);
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;

@@ -400,13 +400,15 @@ __device__ inline bool traverse(const float4* __restrict__ leaf, const int2* __r
     return ok;
 }
 
-template <int PRED, int PREC, bool VIR, bool NT>
+// STAGED: outputs as one staged record per particle (large outputs), else ordinary
+// stores.
+template <int PRED, int PREC, bool VIR, bool STAGED>
 __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restrict__ leaf, const int2* __restrict__ range,
                                                     const int* __restrict__ rope_by_end, const float4* __restrict__ box, int n, float h, uint32_t limit, PairConst c,
                                                     float* __restrict__ fx, float* __restrict__ fy,
                                                     float* __restrict__ fz, float* __restrict__ e,
                                                     double* __restrict__ partials, lj_result* __restrict__ res,
-                                                    int* __restrict__ overflow) {
+                                                    int* __restrict__ overflow, float4* __restrict__ stage) {
     extern __shared__ int lds_list[];
     int* list = lds_list + threadIdx.x;
     // consecutive queries walk nearly the same nodes: keep them on one XCD (own L2)
@@ -429,11 +431,18 @@ __global__ __launch_bounds__(kLbvhBlock) void k_lbvh_force(const float4* __restr
         });
         if (!ok) atomicOr(overflow, 1);
         const uint32_t i = __float_as_uint(pi.w);
-        store_out<NT>(&fx[i], (float)(-c.e24 * (double)acc.ax.value()));
-        store_out<NT>(&fy[i], (float)(-c.e24 * (double)acc.ay.value()));
-        store_out<NT>(&fz[i], (float)(-c.e24 * (double)acc.az.value()));
+        const float ox = (float)(-c.e24 * (double)acc.ax.value());
+        const float oy = (float)(-c.e24 * (double)acc.ay.value());
+        const float oz = (float)(-c.e24 * (double)acc.az.value());
         const double ei = c.e4 * (double)acc.ae.value();
-        if (e) store_out<NT>(&e[i], (float)ei);
+        if constexpr (STAGED) {  // large outputs at shuffled indices: one staged record
+            store_staged(&stage[i], ox, oy, oz, (float)ei);
+        } else {
+            fx[i] = ox;
+            fy[i] = oy;
+            fz[i] = oz;
+            if (e) e[i] = (float)ei;
+        }
         esum = ei;
         vsum = c.e24 * (double)acc.av.value();
     }
@@ -649,10 +658,16 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     const int* rope = ctx->lbvh.rope.as<int>();
     const float4* box = ctx->lbvh.node_box.as<float4>();
     const bool vir = !(p->flags & LJ_FLAG_NO_VIRIAL);
+    // large outputs at shuffled indices go through the staging array
     const bool nt = non_temporal_outputs(n);
+    float4* stage = nullptr;
+    if (nt) {
+        LJ_HIP(ctx, ctx->bin.stage.grow((size_t)n * sizeof(float4)));
+        stage = ctx->bin.stage.as<float4>();
+    }
 #define LJ_LBN(PRED, PREC, VIR, NT)                                                                                  \
     hipLaunchKernelGGL((k_lbvh_force<PRED, PREC, VIR, NT>), dim3(nb), dim3(kLbvhBlock), lds, s, leaf, range, rope, box, \
-                       (int)n, h, lim, c, fx, fy, fz, e, partials, res, ovf)
+                       (int)n, h, lim, c, fx, fy, fz, e, partials, res, ovf, stage)
 #define LJ_LB(PRED, PREC, VIR)                  \
     do {                                        \
         if (nt) LJ_LBN(PRED, PREC, VIR, true);  \
@@ -664,21 +679,25 @@ int lj_lbvh_compute(lj_context* ctx, const float* x, const float* y, const float
     case LJ_PREC_FP32: if (vir) LJ_LB(PRED, LJ_PREC_FP32, true); else LJ_LB(PRED, LJ_PREC_FP32, false); break;    \
     default: if (vir) LJ_LB(PRED, LJ_PREC_FP32_COMP, true); else LJ_LB(PRED, LJ_PREC_FP32_COMP, false); break;    \
     }
-    PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
-    if (n > 0) {
-        switch (p->predicate) {
-        case LJ_PRED_AABB: LJ_LB_PREC(LJ_PRED_AABB); break;
-        case LJ_PRED_CUSTOM: LJ_LB_PREC(LJ_PRED_CUSTOM); break;
-        case LJ_PRED_SPHERE: LJ_LB_PREC(LJ_PRED_SPHERE); break;
-        case LJ_PRED_TRIANGLE: LJ_LB_PREC(LJ_PRED_TRIANGLE); break;
+    {
+        PhaseTimer tf(ctx, LJ_PHASE_FORCE, s);
+        if (n > 0) {
+            switch (p->predicate) {
+            case LJ_PRED_AABB: LJ_LB_PREC(LJ_PRED_AABB); break;
+            case LJ_PRED_CUSTOM: LJ_LB_PREC(LJ_PRED_CUSTOM); break;
+            case LJ_PRED_SPHERE: LJ_LB_PREC(LJ_PRED_SPHERE); break;
+            case LJ_PRED_TRIANGLE: LJ_LB_PREC(LJ_PRED_TRIANGLE); break;
+            }
+        } else {
+            hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true, false>), dim3(1), dim3(kLbvhBlock), lds,
+                               s, leaf, range, rope, box, 0, h, lim, c, fx, fy, fz, e, partials, res, ovf, stage);
         }
-    } else {
-        hipLaunchKernelGGL((k_lbvh_force<LJ_PRED_AABB, LJ_PREC_MIXED, true, false>), dim3(1), dim3(kLbvhBlock), lds, s, leaf,
-                           range, rope, box, 0, h, lim, c, fx, fy, fz, e, partials, res, ovf);
     }
 #undef LJ_LB_PREC
 #undef LJ_LB
 #undef LJ_LBN
+    PhaseTimer to(ctx, LJ_PHASE_OTHER, s);
+    if (stage) split_outputs(stage, (uint32_t)n, fx, fy, fz, e, s);
     finalize_energy(partials, nb, res, s);
     hipLaunchKernelGGL(k_poison_on_overflow, dim3(1), dim3(64), 0, s, (const int*)ovf, res);
     LJ_HIP(ctx, hipGetLastError());

@@ -102,6 +102,7 @@ struct BinWork {
     DevBuf ccounts;      // u32 counts of the cell_edge grid when the force grid is refined
     DevBuf coffsets;     // u32 offsets of the cell_edge grid (ncells+1)
     DevBuf cand;         // u64 per-workgroup partial candidate counts
+    DevBuf stage;        // float4 per particle: outputs staged at the caller index
     int cur = 0;         // which keys buffer holds the sorted keys
     const uint32_t* sorted_keys() const { return keys[cur].as<uint32_t>(); }
 };
@@ -230,6 +231,9 @@ int count_candidates_from(lj_context* ctx, const GridGeom& g, const uint32_t* of
                           hipStream_t s);
 
 void fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
+// fx, fy, fz, e[i] = stage[i] for i < n (e nullable): outputs staged as one float4 per
+// particle at its caller index, copied out with consecutive loads and stores.
+void split_outputs(const float4* stage, uint32_t n, float* fx, float* fy, float* fz, float* e, hipStream_t s);
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 
 // Records a phase's start event on construction and its stop event on destruction

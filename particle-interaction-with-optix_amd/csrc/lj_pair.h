@@ -146,10 +146,11 @@ struct PairSums {
 // where it would displace the neighbour positions the force loops re-read.  Measured
 // (force kernels, ordinary -> non-temporal): C2 1M particles 0.145 -> 0.161 ms, C5 4.2M
 // 0.91 -> 0.88 ms (LBVH 2.65 -> 2.60), C3 16.7M 4.26 -> 4.03 ms, and the batched C4,
-// whose systems each write a range of at most 256 KB, 8.1 -> 9.1 ms.  The kind is a
-// template parameter of the kernels (non_temporal_outputs on the host picks the
-// instance): with a run-time branch between the two forms the compiler merges them into
-// one ordinary store.
+// whose systems each write a range of at most 256 KB, 8.1 -> 9.1 ms.  Beyond the L2s the
+// caller-order outputs are now staged (store_staged, below); non-temporal scalar stores
+// remain for lj_compute_sorted's consecutive ones.  The kind is a template parameter of
+// the kernels (chosen on the host by non_temporal_outputs): with a run-time branch
+// between the two forms the compiler merges them into one ordinary store.
 template <bool NT>
 __device__ inline void store_out(float* p, float v) {
     if constexpr (NT) __builtin_nontemporal_store(v, p);
@@ -159,6 +160,18 @@ __device__ inline void store_out(float* p, float v) {
 // Whether a launch writing n particles' outputs (four floats each, at random indices)
 // should store them non-temporally: when they exceed the chip's L2 (8 XCDs x 4 MiB).
 inline bool non_temporal_outputs(int64_t n) { return (uint64_t)(n > 0 ? n : 0) * 16u > (32ull << 20); }
+
+// Staged outputs.  Beyond the L2s, a particle's four outputs at a shuffled index are
+// four partial writes of four sectors; written instead as one 16-byte record (fx, fy,
+// fz, e) at that index of a staging array, they are one partial write, and
+// split_outputs then copies the staging array into the four caller arrays with
+// consecutive loads and stores.  C3 (16.7M particles, outputs in caller order): force
+// kernel 4.07 -> 3.81 ms with the split pass included, step 4.83 -> 4.61 ms.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ inline void store_staged(float4* p, float a, float b, float c, float d) {
+    const f32x4 v = {a, b, c, d};
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(p));
+}
 
 template <class T>
 __device__ inline T wave_sum(T v) {
