@@ -94,3 +94,35 @@ def test_compute_sorted_other_predicates_and_queues(engine, pred, depth):
         a = out[k][:n].cpu().numpy().view(np.uint32)
         b = ref[k][:n].cpu().numpy()[perm].view(np.uint32)
         assert np.array_equal(a, b), k
+
+
+def test_staged_outputs_without_energies_and_after_a_larger_call(engine):
+    """Outputs beyond the L2s go through a staging array (one record per particle at its
+    caller index, then split into fx, fy, fz, e).  Without the per-particle energies
+    (e = NULL) the forces and the result record are those of the call with them, and a
+    smaller call after a larger one on the same context gets its own outputs (every
+    caller index of the smaller call is staged again): equal, permuted, to
+    lj_compute_sorted's, whose stores go straight to the sorted slots.  The LBVH call
+    shares the staging array and the split."""
+    torch = engine.torch
+    for n, seed in ((3_000_000, 91), (2_300_000, 92)):
+        x, y, z = ljamd.generate("uniform", seed, n)
+        p = ljamd.make_params(0.012)
+        print(f"[staged] n={n}", flush=True)
+        dx, dy, dz = engine.to_device(x, y, z)
+        for lbvh in (False, True):
+            a = engine.compute(dx, dy, dz, p, lbvh=lbvh)
+            b = engine.compute(dx, dy, dz, p, energies=False, lbvh=lbvh)
+            torch.cuda.synchronize()
+            assert b["e"] is None
+            assert a["result"].cpu().numpy().tobytes() == b["result"].cpu().numpy().tobytes()
+            for k in ("fx", "fy", "fz"):
+                assert np.array_equal(a[k].cpu().numpy().view(np.uint32), b[k].cpu().numpy().view(np.uint32))
+            if not lbvh:
+                sx, sy, sz = (t.clone() for t in (dx, dy, dz))
+                s = engine.compute_sorted(sx, sy, sz, p)
+                torch.cuda.synchronize()
+                perm = s["perm"][:n].cpu().numpy().view(np.uint32)
+                for k in ("fx", "fy", "fz", "e"):
+                    assert np.array_equal(a[k].cpu().numpy()[perm].view(np.uint32), s[k].cpu().numpy().view(np.uint32))
+        del dx, dy, dz
