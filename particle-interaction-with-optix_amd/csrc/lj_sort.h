@@ -41,14 +41,16 @@ __device__ inline unsigned rank_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Key + payload of element idx, from positions (first pass) or from the previous pass.
+// Key + payload of element idx, from positions (first pass: the key read from kin,
+// where the first histogram stored it, or computed when kin is null) or from the
+// previous pass.
 template <bool FIRST, class KeyOp>
 __device__ inline void load_elem(uint32_t idx, const float* __restrict__ x, const float* __restrict__ y,
                                  const float* __restrict__ z, const uint32_t* __restrict__ kin,
                                  const float4* __restrict__ pin, const KeyOp& op, uint32_t& key, float4& pay) {
     if constexpr (FIRST) {
         const float px = x[idx], py = y[idx], pz = z[idx];
-        key = op(px, py, pz, idx);
+        key = kin ? kin[idx] : op(px, py, pz, idx);
         pay = make_float4(px, py, pz, __uint_as_float(idx));
     } else {
         key = kin[idx];
@@ -56,10 +58,13 @@ __device__ inline void load_elem(uint32_t idx, const float* __restrict__ x, cons
     }
 }
 
+// The first pass computes the keys and stores them in kstore (input order) for its
+// scatter, which then need not compute them again.
 template <int RB, bool FIRST, class KeyOp>
 __global__ __launch_bounds__(kThreads) void k_hist(const float* __restrict__ x, const float* __restrict__ y,
                                                    const float* __restrict__ z, const uint32_t* __restrict__ kin,
-                                                   uint32_t n, KeyOp op, int shift, uint32_t* __restrict__ hist) {
+                                                   uint32_t n, KeyOp op, int shift, uint32_t* __restrict__ hist,
+                                                   uint32_t* __restrict__ kstore) {
     constexpr int kRadix = 1 << RB;
     __shared__ uint32_t h[kRadix];
     for (int d = threadIdx.x; d < kRadix; d += kThreads) h[d] = 0;
@@ -70,8 +75,12 @@ __global__ __launch_bounds__(kThreads) void k_hist(const float* __restrict__ x, 
         const uint32_t idx = base + r * kThreads + threadIdx.x;
         if (idx < n) {
             uint32_t key;
-            if constexpr (FIRST) key = op(x[idx], y[idx], z[idx], idx);
-            else key = kin[idx];
+            if constexpr (FIRST) {
+                key = op(x[idx], y[idx], z[idx], idx);
+                kstore[idx] = key;
+            } else {
+                key = kin[idx];
+            }
             atomicAdd(&h[(key >> shift) & (kRadix - 1)], 1u);
         }
     }
@@ -316,11 +325,13 @@ int sort_passes(lj_context* ctx, const float* x, const float* y, const float* z,
         float4* pdst = pbuf[(passes - 1 - p) & 1];
         const float4* psrc = pbuf[(passes - p) & 1];
         if (p == 0) {
+            // the keys in input order go to keys[0], which pass 0 does not otherwise use
             hipLaunchKernelGGL((k_hist<RB, true, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
-                               (const uint32_t*)nullptr, (uint32_t)n, op, shift, hist);
+                               (const uint32_t*)nullptr, (uint32_t)n, op, shift, hist, w.keys[0].as<uint32_t>());
         } else {
             hipLaunchKernelGGL((k_hist<RB, false, KeyOp>), dim3(ntiles), dim3(kThreads), 0, s, x, y, z,
-                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), (uint32_t)n, op, shift, hist);
+                               (const uint32_t*)w.keys[kcur].as<uint32_t>(), (uint32_t)n, op, shift, hist,
+                               (uint32_t*)nullptr);
         }
         hipLaunchKernelGGL(k_dig_reduce<RB>, dim3(nchunks), dim3(kRadix), 0, s, (const uint32_t*)hist, ntiles, chunk,
                            part);
@@ -328,10 +339,10 @@ int sort_passes(lj_context* ctx, const float* x, const float* y, const float* z,
         hipLaunchKernelGGL(k_dig_down<RB>, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, chunk,
                            (const uint32_t*)part);
         if (p == 0) {
-            // pass 0 reads positions; it writes keys[1] so that keys[0] stays free
+            // pass 0 reads positions and the stored keys; it writes keys[1]
             hipLaunchKernelGGL((k_scatter<RB, true, KeyOp>), dim3(ntiles), dim3(kSThreads), 0, s, x, y, z,
-                               (const uint32_t*)nullptr, (const float4*)nullptr, (uint32_t)n, op, shift,
-                               (const uint32_t*)hist, w.keys[1].as<uint32_t>(), pdst);
+                               (const uint32_t*)w.keys[0].as<uint32_t>(), (const float4*)nullptr, (uint32_t)n, op,
+                               shift, (const uint32_t*)hist, w.keys[1].as<uint32_t>(), pdst);
             kcur = 1;
         } else {
             hipLaunchKernelGGL((k_scatter<RB, false, KeyOp>), dim3(ntiles), dim3(kSThreads), 0, s, x, y, z,
