@@ -159,7 +159,7 @@ __device__ inline void store_out(float* p, float v) {
 
 // Whether a launch writing n particles' outputs (four floats each, at random indices)
 // should store them non-temporally: when they exceed the chip's L2 (8 XCDs x 4 MiB).
-inline bool non_temporal_outputs(int64_t n) { return (uint64_t)(n > 0 ? n : 0) * 16u > (32ull << 20); }
+constexpr bool non_temporal_outputs(int64_t n) { return (uint64_t)(n > 0 ? n : 0) * 16u > (32ull << 20); }
 
 // Staged outputs.  Beyond the L2s, a particle's four outputs at a shuffled index are
 // four partial writes of four sectors; written instead as one 16-byte record (fx, fy,

@@ -22,6 +22,9 @@ namespace lj {
 
 constexpr size_t kScratchBytes = (size_t)LJ_NOALLOC_SCRATCH_BYTES;
 static_assert(kScratchBytes % 256 == 0, "scratch size is a multiple of the carve alignment");
+// The outputs of the largest input fit in the L2s, so the pipeline stores them directly
+// and never needs the staging array of larger calls.
+static_assert(!(non_temporal_outputs((int64_t)LJ_NOALLOC_MAX_PARTICLES)), "no staged outputs under lj_compute_noalloc");
 
 // The workspace of lj_compute_noalloc: 64 MiB per GPU, enough for the largest input
 // it takes (2^20 particles on a force grid of 2^21 cells needs about 57 MiB).
