@@ -24,7 +24,7 @@ void for_each_buffer(lj_context* ctx, F&& f) {
     LbvhWork& l = ctx->lbvh;
     for (DevBuf* d : {&l.node_box, &l.node_child, &l.range, &l.chunks, &l.rope, &l.flags, &l.bucket_root}) f(*d);
     BatchWork& t = ctx->batch;
-    for (DevBuf* d : {&t.sys, &t.hints, &t.npair, &t.ev, &t.wrec}) f(*d);
+    for (DevBuf* d : {&t.sys, &t.hints, &t.npair, &t.ev, &t.wrec, &t.cand}) f(*d);
     f(ctx->tune_out);
 }
 
@@ -179,6 +179,14 @@ int lj_profile_read(lj_context* ctx, double ms[3], unsigned long long calls[3]) 
 
 const char* lj_version(void) { return "lj_amd 0.1 (gfx950)"; }
 
+static void release_streams(lj_context* ctx) {
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+    ctx->ev_join = ctx->ev_fork = nullptr;
+    ctx->aux = nullptr;
+}
+
 int lj_context_create(lj_context** out, int device) {
     if (!out) return fail(nullptr, LJ_ERR_INVALID_ARG, "out is NULL");
     *out = nullptr;
@@ -205,6 +213,20 @@ int lj_context_create(lj_context** out, int device) {
         delete ctx;
         return sr;
     }
+    // the second stream and its fork/join events (default priority; the caller's stream
+    // may be the null stream, so the second one does not synchronise with it implicitly)
+    {
+        hipError_t se = hipSetDevice(device);
+        if (se == hipSuccess) se = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
+        if (se == hipSuccess) se = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+        if (se == hipSuccess) se = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+        if (se != hipSuccess) {
+            fail(nullptr, LJ_ERR_HIP, "second stream: %s", hipGetErrorString(se));
+            release_streams(ctx);
+            delete ctx;
+            return LJ_ERR_HIP;
+        }
+    }
     // test hook: LJ_TEST_FAIL_ALLOC_AFTER=k lets k more device allocations succeed
     const char* inj = std::getenv("LJ_TEST_FAIL_ALLOC_AFTER");
     g_alloc_budget.store(inj && *inj ? std::strtoll(inj, nullptr, 10) : -1, std::memory_order_relaxed);
@@ -225,6 +247,7 @@ int lj_context_destroy(lj_context* ctx) {
     if (!ctx) return LJ_OK;
     (void)hipSetDevice(ctx->device);
     release_all(ctx);
+    release_streams(ctx);
     delete ctx;
     return LJ_OK;
 }

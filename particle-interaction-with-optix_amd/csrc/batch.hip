@@ -86,13 +86,14 @@ __global__ __launch_bounds__(256) void k_batch_coarse_counts(const uint32_t* __r
 }
 
 // 27-cell candidates per system on the cell_edge grids (offsets coff over the
-// concatenated cell_edge cells), ComputeNbInteractions semantics (cudaLJ.cu:255-297).
-// A wave's cells usually belong to one system: it then adds its sum with one atomic.
+// concatenated cell_edge cells), ComputeNbInteractions semantics (cudaLJ.cu:255-297),
+// added into cand[system].  A wave's cells usually belong to one system: it then adds
+// its sum with one atomic.
 __global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __restrict__ coff, uint32_t nccells,
                                                           const SysDesc* __restrict__ sys,
                                                           const uint32_t* __restrict__ ccbase,
                                                           const uint32_t* __restrict__ cchint, uint32_t nsys,
-                                                          lj_result* __restrict__ res) {
+                                                          unsigned long long* __restrict__ cand) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t s = 0xffffffffu;
     unsigned long long v = 0;
@@ -121,9 +122,9 @@ __global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __rest
     const uint32_t s0 = __shfl(s, 0, kWaveSize);
     if (__all(s == s0 || s == 0xffffffffu)) {
         v = wave_sum(v);
-        if (__lane_id() == 0 && v) atomicAdd(&res[s0].n_candidates, v);
+        if (__lane_id() == 0 && v) atomicAdd(&cand[s0], v);
     } else if (v) {
-        atomicAdd(&res[s].n_candidates, v);
+        atomicAdd(&cand[s], v);
     }
 }
 
@@ -233,11 +234,13 @@ __global__ __launch_bounds__(kWaveSize) void k_batch_force_shared(
 
 // One block per system: FP64 totals over its waves in slot order -- the WaveTotals of
 // the waves it fills alone, the per-slot values of its slots in the (at most two)
-// waves it shares with other systems.
+// waves it shares with other systems -- and its candidate count from cand.
 __global__ __launch_bounds__(256) void k_batch_reduce(const SysDesc* __restrict__ sys,
                                                       const WaveTotals* __restrict__ wrec,
                                                       const double2* __restrict__ ev,
-                                                      const uint32_t* __restrict__ npair, lj_result* __restrict__ res) {
+                                                      const uint32_t* __restrict__ npair,
+                                                      const unsigned long long* __restrict__ cand,
+                                                      lj_result* __restrict__ res) {
     const SysDesc& d = sys[blockIdx.x];
     double a = 0.0, b = 0.0;
     unsigned long long np = 0;
@@ -267,6 +270,7 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const SysDesc* __restrict_
         res[blockIdx.x].e_total = 0.5 * a;
         res[blockIdx.x].virial = 0.5 * b;
         res[blockIdx.x].n_pairs = t;
+        res[blockIdx.x].n_candidates = cand[blockIdx.x];
     }
 }
 
@@ -406,21 +410,30 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
         LJ_TRY(cell_offsets_from_sorted(ctx, n, ncells, s));
         off = B.offsets.as<uint32_t>();
     }
+    // The candidate counts need only the cell offsets: they run on the context's second
+    // stream beside the force pass, into an array of their own that the final
+    // reduction copies into the results after the join.
+    LJ_HIP(ctx, W.cand.grow((size_t)nsys * sizeof(unsigned long long)));
+    unsigned long long* cand = W.cand.as<unsigned long long>();
     {
-        PhaseTimer to(ctx, LJ_PHASE_OTHER, s);
+        const hipStream_t sc = ctx->aux;
+        LJ_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
+        LJ_HIP(ctx, hipStreamWaitEvent(sc, ctx->ev_fork, 0));
+        fill_u64(cand, (uint32_t)nsys, 0ull, sc);
         const uint32_t* coff = off;
         if (sh.refine > 1 || sh.refine_x > 1) {
             LJ_HIP(ctx, B.ccounts.grow((size_t)nccells * 4));
             LJ_HIP(ctx, B.coffsets.grow(((size_t)nccells + 1) * 4));
-            hipLaunchKernelGGL(k_batch_coarse_counts, dim3(div_up(nccells, 256)), dim3(256), 0, s, (const uint32_t*)off,
-                               nccells, dsys, dccbase, hccbase, (uint32_t)nsys, sh.refine_x, sh.refine,
-                               B.ccounts.as<uint32_t>());
+            hipLaunchKernelGGL(k_batch_coarse_counts, dim3(div_up(nccells, 256)), dim3(256), 0, sc,
+                               (const uint32_t*)off, nccells, dsys, dccbase, hccbase, (uint32_t)nsys, sh.refine_x,
+                               sh.refine, B.ccounts.as<uint32_t>());
             uint32_t* co = B.coffsets.as<uint32_t>();
-            LJ_TRY(scan_u32(ctx, B.ccounts.as<uint32_t>(), co, nccells, co + nccells, s));
+            LJ_TRY(scan_u32(ctx, B.ccounts.as<uint32_t>(), co, nccells, co + nccells, sc));
             coff = co;
         }
-        hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(nccells, 256)), dim3(256), 0, s, coff, nccells, dsys,
-                           dccbase, hccbase, (uint32_t)nsys, results);
+        hipLaunchKernelGGL(k_batch_candidates, dim3(div_up(nccells, 256)), dim3(256), 0, sc, coff, nccells, dsys,
+                           dccbase, hccbase, (uint32_t)nsys, cand);
+        LJ_HIP(ctx, hipEventRecord(ctx->ev_join, sc));
     }
     double2* ev = W.ev.as<double2>();
     uint32_t* npair = W.npair.as<uint32_t>();
@@ -469,8 +482,9 @@ extern "C" int lj_compute_batched(lj_context* ctx, const lj_system* sys, int32_t
     }
     {
         PhaseTimer to(ctx, LJ_PHASE_OTHER, s);
+        LJ_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join, 0));
         hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)nsys), dim3(256), 0, s, dsys, (const WaveTotals*)wrec,
-                           (const double2*)ev, (const uint32_t*)npair, results);
+                           (const double2*)ev, (const uint32_t*)npair, (const unsigned long long*)cand, results);
     }
     LJ_HIP(ctx, hipGetLastError());
     return LJ_OK;

@@ -125,6 +125,7 @@ struct BatchWork {
     DevBuf npair;         // u32 accepted-pair count of each sorted slot in a wave shared by systems
     DevBuf ev;            // double2 energy and virial of each sorted slot in a wave shared by systems
     DevBuf wrec;          // one WaveTotals per wave of 64 sorted slots
+    DevBuf cand;          // u64 candidate count per system, counted on the second stream
     std::vector<uint8_t> plan;  // host copy of the plan last uploaded to sys
 };
 
@@ -151,6 +152,11 @@ struct lj_context {
     lj::BatchWork batch;
     lj::DevBuf tune_out;  // outputs of the autotuner's timed calls
     void* scratch = nullptr;  // device address of this device's lj_compute_noalloc array
+    // A second stream (default priority) for work that can run beside the force pass
+    // (the batched call's candidate counts), forked from and joined into the caller's
+    // stream with these two events.
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace lj {

@@ -357,6 +357,42 @@ def test_deterministic_bits(engine):
     assert ra.tobytes() == rb.tobytes()
 
 
+@pytest.mark.parametrize("name", ["c1_uniform_100k", "c3_sub_big"])
+def test_compute_graph_replays_match_eager(engine, name):
+    """lj_compute captured in a graph replays to the eager bits, with unrelated eager
+    work between the replays (small outputs stored directly; large ones staged and
+    split)."""
+    torch = engine.torch
+    if name == "c3_sub_big":  # outputs beyond the L2s: staged, then split
+        x, y, z = ljamd.generate("uniform", 1236, 2_500_000)
+        p = ljamd.make_params(0.01)
+    else:
+        (x, y, z), p = _case(name)
+    print(f"[graph] {name} n={x.size}", flush=True)
+    dx, dy, dz = engine.to_device(x, y, z)
+    out = engine.compute(dx, dy, dz, p)
+    torch.cuda.synchronize()
+    eager = {k: out[k].clone() for k in ("fx", "fy", "fz", "e", "result")}
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        engine.compute(dx, dy, dz, p, out=out)
+    torch.cuda.current_stream().wait_stream(s)
+    with torch.cuda.graph(g):
+        engine.compute(dx, dy, dz, p, out=out)
+    for rep in range(3):
+        for k in ("fx", "fy", "fz", "e", "result"):
+            out[k].zero_()
+        engine.compute(dx[:500], dy[:500], dz[:500], ljamd.make_params(0.1))  # unrelated eager work
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k in ("fx", "fy", "fz", "e", "result"):
+            assert torch.equal(out[k], eager[k]), (rep, k)
+    del g
+
+
 def test_invalid_params_rejected(engine):
     x, y, z = ljamd.generate("uniform", 1, 100)
     dx, dy, dz = engine.to_device(x, y, z)
