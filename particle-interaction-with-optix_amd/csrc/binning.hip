@@ -151,63 +151,75 @@ __device__ inline void fill_cells(uint32_t a, uint32_t b, uint32_t v, uint32_t* 
 
 // The cells a wave writes, [A, E), are contiguous (the runs starting at its slots
 // cover (key[k - 1], key[k]] one after the other, in lane order).  With STAGE (grids
-// of two or more cells per particle, chosen by the host), a wave whose range holds
-// kStageMin to kStage cells -- several per lane, so that lanes would loop over short
-// gaps with scattered stores -- fills a copy in LDS (short gaps lane by lane, longer
-// ones by the whole wave) and stores it with consecutive lanes on consecutive cells;
-// otherwise the cells are written in place as fill_cells does.  (On grids of about one
-// cell per particle the stores are already consecutive, and the staging code alone
-// made the kernel slower.)
+// of 1.5 or more cells per particle, chosen by the host), a wave whose range holds
+// kStageMin to kStage cells -- about one per lane or more, so that lanes would loop
+// over short gaps with scattered stores -- fills a copy in LDS (short gaps lane by
+// lane, longer ones by the whole wave) and stores it with consecutive lanes on
+// consecutive cells; otherwise the cells are written in place as fill_cells does.
+// (On grids of about one cell per particle the stores are already consecutive, and
+// the staging code alone made the kernel slower.)  Each workgroup takes kOffChunks
+// consecutive 256-key chunks and loads all their keys first: one key per thread left
+// the kernel waiting on its loads (C3 on 32M cells 80 -> 50 us, on 16M cells 52 -> 39
+// us, the C4 batch 252 -> 182 us).
 constexpr uint32_t kStage = 1024;
-constexpr uint32_t kStageMin = 2 * kWave;
+constexpr uint32_t kStageMin = kWave;
+constexpr int kOffChunks = 4;
 
 template <bool STAGE>
 __global__ __launch_bounds__(256) void k_offsets(const uint32_t* __restrict__ skeys, uint32_t n, uint32_t ncells,
                                                  uint32_t* __restrict__ off, GapPiece* __restrict__ gaps,
                                                  uint32_t* __restrict__ ngaps) {
     __shared__ uint32_t stage[STAGE ? 256 / kWave : 1][STAGE ? kStage : 1];
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t a = 0, b = 0, v = 0, key = 0;  // the cells (previous key, key] take slot k
-    if (k < n) {
-        key = skeys[k];
-        if (k == 0) {
-            b = key + 1u;
-        } else {
-            const uint32_t prev = skeys[k - 1];
-            if (prev != key) {
-                a = prev + 1u;
+    // all the workgroup's keys are loaded before any cell is written
+    uint32_t kk[kOffChunks], pk[kOffChunks];
+#pragma unroll
+    for (int u = 0; u < kOffChunks; ++u) {
+        const uint32_t k = (blockIdx.x * kOffChunks + u) * blockDim.x + threadIdx.x;
+        kk[u] = k < n ? skeys[k] : 0u;
+        pk[u] = (k < n && k > 0) ? skeys[k - 1] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kOffChunks; ++u) {
+        const uint32_t k = (blockIdx.x * kOffChunks + u) * blockDim.x + threadIdx.x;
+        uint32_t a = 0, b = 0, v = 0;  // the cells (previous key, key] take slot k
+        const uint32_t key = kk[u];
+        if (k < n) {
+            if (k == 0) {
+                b = key + 1u;
+            } else if (pk[u] != key) {
+                a = pk[u] + 1u;
                 b = key + 1u;
                 v = k;
             }
         }
-    }
-    uint32_t A = 0u, E = 0u;
-    if constexpr (STAGE) {
-        const uint64_t runs = __ballot(b > a);
-        if (runs) {
-            A = __builtin_amdgcn_readlane(a, __builtin_ctzll(runs));
-            E = __builtin_amdgcn_readlane(b, 63 - __builtin_clzll(runs));
+        uint32_t A = 0u, E = 0u;
+        if constexpr (STAGE) {
+            const uint64_t runs = __ballot(b > a);
+            if (runs) {
+                A = __builtin_amdgcn_readlane(a, __builtin_ctzll(runs));
+                E = __builtin_amdgcn_readlane(b, 63 - __builtin_clzll(runs));
+            }
         }
-    }
-    if (STAGE && E - A >= kStageMin && E - A <= kStage) {
-        uint32_t* st = stage[threadIdx.x / kWave];
-        const uint32_t len = b - a;
-        if (len <= kGapLane)
-            for (uint32_t c = a; c < b; ++c) st[c - A] = v;
-        for (uint64_t m = __ballot(len > kGapLane); m; m &= m - 1) {
-            const int l = __builtin_ctzll(m);
-            const uint32_t la = __builtin_amdgcn_readlane(a, l), lb = __builtin_amdgcn_readlane(b, l);
-            const uint32_t lv = __builtin_amdgcn_readlane(v, l);
-            for (uint32_t c = la + lane_id(); c < lb; c += kWave) st[c - A] = lv;
+        if (STAGE && E - A >= kStageMin && E - A <= kStage) {
+            uint32_t* st = stage[threadIdx.x / kWave];
+            const uint32_t len = b - a;
+            if (len <= kGapLane)
+                for (uint32_t c = a; c < b; ++c) st[c - A] = v;
+            for (uint64_t m = __ballot(len > kGapLane); m; m &= m - 1) {
+                const int l = __builtin_ctzll(m);
+                const uint32_t la = __builtin_amdgcn_readlane(a, l), lb = __builtin_amdgcn_readlane(b, l);
+                const uint32_t lv = __builtin_amdgcn_readlane(v, l);
+                for (uint32_t c = la + lane_id(); c < lb; c += kWave) st[c - A] = lv;
+            }
+            for (uint32_t c = lane_id(); c < E - A; c += kWave) off[A + c] = st[c];
+        } else {
+            fill_cells(a, b, v, off, gaps, ngaps);
         }
-        for (uint32_t c = lane_id(); c < E - A; c += kWave) off[A + c] = st[c];
-    } else {
-        fill_cells(a, b, v, off, gaps, ngaps);
-    }
-    // the cells above the last key take n (the wave holding slot n - 1)
-    if (__any(k == n - 1)) {
-        const bool last = k == n - 1;
-        fill_cells(last ? key + 1u : 0u, last ? ncells + 1u : 0u, n, off, gaps, ngaps);
+        // the cells above the last key take n (the wave holding slot n - 1)
+        if (__any(k == n - 1)) {
+            const bool last = k == n - 1;
+            fill_cells(last ? key + 1u : 0u, last ? ncells + 1u : 0u, n, off, gaps, ngaps);
+        }
     }
 }
 
@@ -393,11 +405,11 @@ int cell_offsets_from_sorted(lj_context* ctx, int64_t n, uint32_t ncells, hipStr
     uint32_t* ngaps = w.gaps.as<uint32_t>();
     GapPiece* gaps = reinterpret_cast<GapPiece*>(w.gaps.as<uint8_t>() + sizeof(uint32_t) * 4);
     fill_u32(ngaps, 1, 0u, s);
-    if ((uint64_t)ncells >= 2 * (uint64_t)n)
-        hipLaunchKernelGGL(k_offsets<true>, dim3(div_up(n, 256)), dim3(256), 0, s, w.sorted_keys(), (uint32_t)n, ncells, off,
+    if (2 * (uint64_t)ncells >= 3 * (uint64_t)n)
+        hipLaunchKernelGGL(k_offsets<true>, dim3(div_up(n, 256 * kOffChunks)), dim3(256), 0, s, w.sorted_keys(), (uint32_t)n, ncells, off,
                        gaps, ngaps);
     else
-        hipLaunchKernelGGL(k_offsets<false>, dim3(div_up(n, 256)), dim3(256), 0, s, w.sorted_keys(), (uint32_t)n, ncells, off,
+        hipLaunchKernelGGL(k_offsets<false>, dim3(div_up(n, 256 * kOffChunks)), dim3(256), 0, s, w.sorted_keys(), (uint32_t)n, ncells, off,
                        gaps, ngaps);
     const unsigned nb = (unsigned)(cap < kGapBlocks ? cap : kGapBlocks);
     hipLaunchKernelGGL(k_fill_gaps, dim3(nb), dim3(256), 0, s, (const GapPiece*)gaps, (const uint32_t*)ngaps, off);

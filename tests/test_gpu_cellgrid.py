@@ -61,15 +61,23 @@ def _case(name):
         x = ((cells + 0.5) * rc).astype(np.float32)
         y = np.full(x.size, 0.5 * rc, np.float32)
         return (x, y, y.copy()), ljamd.make_params(rc, box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, rc, rc))
+    if name == "sparse_25bit_keys":
+        # 300^3 = 27,000,000 cells: a 25-bit key, sorted in passes of 8, 8 and 9 bits
+        x, y, z = ljamd.generate("uniform", 13, 200_000)
+        return (x, y, z), ljamd.make_params(1.0 / 300)
     if name == "sparse_26bit_keys":
-        # 333^3 = 36,926,037 cells: a 26-bit key, sorted in three 9-bit passes
+        # 333^3 = 36,926,037 cells: a 26-bit key, sorted in passes of 8, 9 and 9 bits
         x, y, z = ljamd.generate("uniform", 12, 200_000)
         return (x, y, z), ljamd.make_params(0.003)
+    if name == "sparse_27bit_keys":
+        # 450^3 = 91,125,000 cells: a 27-bit key, sorted in three 9-bit passes
+        x, y, z = ljamd.generate("uniform", 14, 200_000)
+        return (x, y, z), ljamd.make_params(1.0 / 450)
     raise KeyError(name)
 
 
 BIN_CASES = ["c1_uniform_100k", "c2_surface_200k", "c5_clusters_64k", "tiny_7", "single", "empty", "edges", "one_cell_crowd",
-             "sparse_26bit_keys", "gap_classes"]
+             "sparse_25bit_keys", "sparse_26bit_keys", "sparse_27bit_keys", "gap_classes"]
 
 
 @pytest.mark.parametrize("name", BIN_CASES)
