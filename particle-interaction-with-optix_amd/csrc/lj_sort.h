@@ -71,20 +71,25 @@ __global__ __launch_bounds__(kThreads) void k_hist(const float* __restrict__ x, 
     for (int d = threadIdx.x; d < kRadix; d += kThreads) h[d] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kTile;
+    // every key of the thread first, then the counts: LDS atomics between the loads
+    // would keep one load in flight per wave
+    uint32_t key[kItems];
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         const uint32_t idx = base + r * kThreads + threadIdx.x;
+        key[r] = 0;
         if (idx < n) {
-            uint32_t key;
             if constexpr (FIRST) {
-                key = op(x[idx], y[idx], z[idx], idx);
-                kstore[idx] = key;
+                key[r] = op(x[idx], y[idx], z[idx], idx);
+                kstore[idx] = key[r];
             } else {
-                key = kin[idx];
+                key[r] = kin[idx];
             }
-            atomicAdd(&h[(key >> shift) & (kRadix - 1)], 1u);
         }
     }
+#pragma unroll
+    for (int r = 0; r < kItems; ++r)
+        if (base + r * kThreads + threadIdx.x < n) atomicAdd(&h[(key[r] >> shift) & (kRadix - 1)], 1u);
     __syncthreads();
     for (int d = threadIdx.x; d < kRadix; d += kThreads) hist[(uint64_t)blockIdx.x * kRadix + d] = h[d];
 }
