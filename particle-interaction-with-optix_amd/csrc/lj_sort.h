@@ -38,6 +38,8 @@ constexpr int kWaveKeys = kSItems * kWave;  // the contiguous keys of one wave i
 static_assert(kSItems * kSThreads == kTile, "scatter shape");
 constexpr uint32_t kMaxChunks = 256;       // row groups of the offset scan
 
+typedef float vf4 __attribute__((ext_vector_type(4)));
+
 __device__ inline unsigned rank_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
@@ -71,20 +73,32 @@ __global__ __launch_bounds__(kThreads) void k_hist(const float* __restrict__ x, 
     for (int d = threadIdx.x; d < kRadix; d += kThreads) h[d] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kTile;
-    // every key of the thread first, then the counts: LDS atomics between the loads
-    // would keep one load in flight per wave
+    // Every load of the thread is issued before any key is used: items past n load the
+    // last element instead (n >= 1) and are not counted, so no branch separates the
+    // loads (with a guard or an LDS atomic between them a wave had one item's loads in
+    // flight at a time).
     uint32_t key[kItems];
+    if constexpr (FIRST) {
+        float px[kItems], py[kItems], pz[kItems];
 #pragma unroll
-    for (int r = 0; r < kItems; ++r) {
-        const uint32_t idx = base + r * kThreads + threadIdx.x;
-        key[r] = 0;
-        if (idx < n) {
-            if constexpr (FIRST) {
-                key[r] = op(x[idx], y[idx], z[idx], idx);
-                kstore[idx] = key[r];
-            } else {
-                key[r] = kin[idx];
-            }
+        for (int r = 0; r < kItems; ++r) {
+            const uint32_t idx = base + r * kThreads + threadIdx.x;
+            const uint32_t ic = idx < n ? idx : n - 1;
+            px[r] = x[ic];
+            py[r] = y[ic];
+            pz[r] = z[ic];
+        }
+#pragma unroll
+        for (int r = 0; r < kItems; ++r) {
+            const uint32_t idx = base + r * kThreads + threadIdx.x;
+            key[r] = op(px[r], py[r], pz[r], idx < n ? idx : n - 1);
+            if (idx < n) kstore[idx] = key[r];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kItems; ++r) {
+            const uint32_t idx = base + r * kThreads + threadIdx.x;
+            key[r] = kin[idx < n ? idx : n - 1];
         }
     }
 #pragma unroll
@@ -190,7 +204,12 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
     static_assert(kRadix <= kSThreads, "one thread per digit combines the waves' counts");
     __shared__ uint32_t skey[kTile];
     __shared__ float4 spay[kTile];
-    __shared__ uint32_t wcnt[kSWaves][kRadix];  // per-wave digit counts, then per-wave first slots
+    // per-wave digit counts, then per-wave first slots, as 16-bit halves of 32-bit words
+    // (digits d and d ^ 1 share word d / 2; every value is at most kTile), so that three
+    // workgroups of 8-bit digits fit in a CU's LDS instead of two
+    __shared__ uint32_t wcnt[kSWaves][kRadix / 2];
+    uint16_t* const wc16 = reinterpret_cast<uint16_t*>(&wcnt[0][0]);  // [wave * kRadix + d]
+    static_assert(kTile < 65536, "16-bit counters");
     __shared__ uint32_t dstart[kRadix];         // first staging slot of each digit in this tile
     __shared__ uint32_t gbase[kRadix];          // first output slot of each digit for this tile
     __shared__ uint32_t wsum[kRadix / kWave];
@@ -199,24 +218,31 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
     const uint32_t base = tile * kTile;
     const uint32_t count = n - base < (uint32_t)kTile ? n - base : (uint32_t)kTile;
 
-#pragma unroll
-    for (int q = 0; q < kRadix / kWave; ++q) wcnt[wave][q * kWave + lane] = 0;
-    if (tid < (unsigned)kRadix) gbase[tid] = hist[(uint64_t)tile * kRadix + tid];
-
     uint32_t key[kSItems];
     float4 pay[kSItems];
     uint32_t rank[kSItems];
+    // Every element load is issued first, with no branch between them: a lane past the
+    // tile's end loads the tile's last element (count >= 1) and is never staged, so its
+    // values are defined.  (With guarded loads a wave had one element's loads in flight
+    // at a time, and the digit table's row, loaded before them, was waited for first.)
+    // The first pass reads the keys its histogram stored (kin is never null here).
 #pragma unroll
     for (int r = 0; r < kSItems; ++r) {
         const uint32_t j = wave * kWaveKeys + r * kWave + lane;
-        if (j < count) load_elem<FIRST>(base + j, x, y, z, kin, pin, op, key[r], pay[r]);
-        else {  // a lane past the tile's end: defined values, never staged
-            // (with the payload left unset here the compiler kept pay[] in scratch, 48
-            // bytes per lane, and the later passes' scatter took 0.1 ms longer on C3)
-            key[r] = 0;
-            pay[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const uint32_t idx = base + (j < count ? j : count - 1);
+        key[r] = kin[idx];
+        if constexpr (FIRST) {
+            pay[r] = make_float4(x[idx], y[idx], z[idx], __uint_as_float(idx));
+        } else {
+            // as a native vector: a float4 struct copied here went through a private
+            // (scratch) copy, 48 bytes per lane with the keys and ranks
+            const vf4 v = reinterpret_cast<const vf4*>(pin)[idx];
+            pay[r] = make_float4(v.x, v.y, v.z, v.w);
         }
     }
+#pragma unroll
+    for (int q = 0; q < kRadix / 2 / kWave; ++q) wcnt[wave][q * kWave + lane] = 0;
+    if (tid < (unsigned)kRadix) gbase[tid] = hist[(uint64_t)tile * kRadix + tid];
     // Stable ranks inside the wave: the lowest lane of each digit group adds the group's
     // size to the wave's counter with an LDS atomic (the wave's own counters, so no
     // other wave touches them) and hands the old value to its group.
@@ -233,7 +259,10 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
         }
         const unsigned rk = rank_below(peers);
         uint32_t before = 0;
-        if (valid && rk == 0) before = atomicAdd(&wcnt[wave][d], (uint32_t)__popcll(peers));
+        if (valid && rk == 0) {
+            const uint32_t half = 16u * (d & 1u);
+            before = (atomicAdd(&wcnt[wave][d >> 1], (uint32_t)__popcll(peers) << half) >> half) & 0xffffu;
+        }
         const int leader = valid ? __builtin_ctzll(peers) : (int)lane;
         before = __shfl(before, leader, kWave);
         rank[r] = before + rk;
@@ -246,7 +275,7 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
     if (tid < (unsigned)kRadix) {
 #pragma unroll
         for (int w = 0; w < kSWaves; ++w) {
-            c[w] = wcnt[w][tid];
+            c[w] = wc16[w * kRadix + tid];
             tot += c[w];
         }
         incl = tot;
@@ -264,7 +293,7 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
         dstart[tid] = run;
 #pragma unroll
         for (int w = 0; w < kSWaves; ++w) {
-            wcnt[w][tid] = run;
+            wc16[w * kRadix + tid] = (uint16_t)run;
             run += c[w];
         }
     }
@@ -273,7 +302,7 @@ __global__ __launch_bounds__(kSThreads) void k_scatter(const float* __restrict__
     for (int r = 0; r < kSItems; ++r) {
         if (wave * kWaveKeys + r * kWave + lane < count) {
             const uint32_t d = (key[r] >> shift) & (kRadix - 1);
-            const uint32_t slot = wcnt[wave][d] + rank[r];
+            const uint32_t slot = wc16[wave * kRadix + d] + rank[r];
             skey[slot] = key[r];
             spay[slot] = pay[r];
         }
