@@ -98,26 +98,33 @@ __global__ __launch_bounds__(256) void k_batch_candidates(const uint32_t* __rest
     uint32_t s = 0xffffffffu;
     unsigned long long v = 0;
     if (c < nccells) {
+        const uint32_t c0 = coff[c], c1 = coff[c + 1];
         s = find_sys(ccbase, cchint, nsys, c);
-        const uint32_t nh = coff[c + 1] - coff[c];
-        if (nh) {
-            const GridGeom& g = sys[s].cg;
-            const uint32_t base = sys[s].ccell_base;
-            int32_t cx, cy, cz;
-            cell_coords(g, c - base, cx, cy, cz);
-            const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
-            const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
-            unsigned long long sum = 0;
-            for (int32_t zz = cz - 1; zz <= cz + 1; ++zz) {
-                if (zz < 0 || zz >= gz) continue;
-                for (int32_t yy = cy - 1; yy <= cy + 1; ++yy) {
-                    if (yy < 0 || yy >= gy) continue;
-                    const uint32_t row = base + (uint32_t)((zz * gy + yy) * gx);
-                    sum += coff[row + x1 + 1] - coff[row + x0];
-                }
-            }
-            v = (unsigned long long)nh * (sum - 1);
+        const GridGeom& g = sys[s].cg;
+        const uint32_t base = sys[s].ccell_base;
+        int32_t cx, cy, cz;
+        cell_coords(g, c - base, cx, cy, cz);
+        const int32_t gx = g.g[0], gy = g.g[1], gz = g.g[2];
+        const int32_t x0 = cx > 0 ? cx - 1 : 0, x1 = cx < gx - 1 ? cx + 1 : gx - 1;
+        // The nine rows' ranges loaded at once, with no branch between the loads: a row
+        // outside the grid reads the cell's own row and counts zero.  (Loaded row by row
+        // under the bounds tests, the loads went out one row at a time.)
+        uint32_t lo[9], hi[9];
+        bool in[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int32_t zz = cz + q / 3 - 1, yy = cy + q % 3 - 1;
+            in[q] = zz >= 0 && zz < gz && yy >= 0 && yy < gy;
+            const uint32_t row = base + (uint32_t)(((in[q] ? zz : cz) * gy + (in[q] ? yy : cy)) * gx);
+            lo[q] = coff[row + x0];
+            hi[q] = coff[row + x1 + 1];
         }
+        unsigned long long sum = 0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+            if (in[q]) sum += hi[q] - lo[q];
+        const uint32_t nh = c1 - c0;
+        if (nh) v = (unsigned long long)nh * (sum - 1);
     }
     const uint32_t s0 = __shfl(s, 0, kWaveSize);
     if (__all(s == s0 || s == 0xffffffffu)) {

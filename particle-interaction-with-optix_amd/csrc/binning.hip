@@ -56,12 +56,18 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const TI* __restri
                                                              TS* __restrict__ partials) {
     __shared__ TS wsum[kScanThreads / kWave];
     const uint32_t base = blockIdx.x * kScanTile;
-    TS s = 0;
+    // every load issued before any is used: indices past n read the last value (n >= 1)
+    // and count zero (a guarded load per item kept one in flight at a time)
+    TI v[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t idx = base + k * kScanThreads + threadIdx.x;
-        if (idx < n) s += (TS)in[idx];
+        v[k] = in[idx < n ? idx : n - 1];
     }
+    TS s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k * kScanThreads + threadIdx.x < n) s += (TS)v[k];
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, kWave);
     if (lane_id() == 0) wsum[threadIdx.x / kWave] = s;
@@ -82,10 +88,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI* in, TS* ou
     __shared__ TS wsum[kScanThreads / kWave];
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     TS v[kScanItems];
+    TI raw[kScanItems];
+    // A thread's eight consecutive 32-bit values as two 16-byte loads when they lie
+    // inside n and in is 16-byte aligned (with eight 4-byte loads a wave's load touched
+    // 16 lines); otherwise one clamped load each, as in k_scan_reduce.
+    constexpr bool kVec = sizeof(TI) == 4 && kScanItems == 8;
+    const bool vec = kVec && base + kScanItems <= n && ((uintptr_t)in & 15u) == 0;
+    if (vec) {
+        const uint4 a = *reinterpret_cast<const uint4*>(in + base);
+        const uint4 b = *reinterpret_cast<const uint4*>(in + base + 4);
+        const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) raw[k] = (TI)w[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) raw[k] = in[base + k < n ? base + k : n - 1];
+    }
     TS s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < n) ? (TS)in[base + k] : (TS)0;
+        v[k] = base + k < n ? (TS)raw[k] : (TS)0;
         s += v[k];
     }
     const TS incl = wave_incl_scan_t(s);
@@ -95,10 +117,21 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI* in, TS* ou
     TS woff = block_off ? block_off[blockIdx.x] : (TS)0;
     for (unsigned q = 0; q < w; ++q) woff += wsum[q];
     TS run = woff + incl - s;
+    if (kVec && sizeof(TS) == 4 && vec && ((uintptr_t)out & 15u) == 0) {
+        uint32_t o[kScanItems];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < n) out[base + k] = run;
-        run += v[k];
+        for (int k = 0; k < kScanItems; ++k) {
+            o[k] = (uint32_t)run;
+            run += v[k];
+        }
+        *reinterpret_cast<uint4*>(out + base) = make_uint4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<uint4*>(out + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (base + k < n) out[base + k] = run;
+            run += v[k];
+        }
     }
     if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) *total = run;
 }
