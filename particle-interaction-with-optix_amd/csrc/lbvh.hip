@@ -282,15 +282,25 @@ __global__ __launch_bounds__(kNodeBoundsBlock) void k_node_bounds(const float4* 
     __shared__ int todo[kNodeBoundsBlock / kWaveSize][kNodesPerWave];
     int* const list = todo[threadIdx.x / kWaveSize];
     const int first = (int)((blockIdx.x * (kNodeBoundsBlock / kWaveSize) + threadIdx.x / kWaveSize) * kNodesPerWave);
+    // Every node's range and mark are loaded before any is tested (a node past n - 2
+    // reads node 0's, n >= 2, and is not listed): tested as they came in, the loads went
+    // out one group of 64 nodes at a time.  The root's mark is never written (it has no
+    // parent); it is read here but not used.
+    constexpr int kGroups = kNodesPerWave / kWaveSize;
+    int2 rr[kGroups];
+    uint8_t mark[kGroups];
+#pragma unroll
+    for (int u = 0; u < kGroups; ++u) {
+        const int i = first + u * kWaveSize + (int)__lane_id();
+        const int ic = i < n - 1 ? i : 0;
+        rr[u] = range[ic];
+        mark[u] = bucket_root[ic];
+    }
     unsigned cnt = 0;  // wave-uniform
 #pragma unroll
-    for (int u = 0; u < kNodesPerWave / kWaveSize; ++u) {
+    for (int u = 0; u < kGroups; ++u) {
         const int i = first + u * kWaveSize + (int)__lane_id();
-        bool need = false;
-        if (i < n - 1) {
-            const int2 r = range[i];
-            need = i == 0 || r.y - r.x + 1 > kBucket || bucket_root[i];
-        }
+        const bool need = i < n - 1 && (i == 0 || rr[u].y - rr[u].x + 1 > kBucket || mark[u]);
         const uint64_t m = __ballot(need);
         if (need) list[cnt + sortk::rank_below(m)] = i;
         cnt += (unsigned)__popcll(m);
